@@ -1,0 +1,124 @@
+// Internal runtime of the bclx library: handle, error reporting, device buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/bclx.h"
+
+namespace bclx {
+
+void set_error(const std::string& msg);
+
+struct Error {
+  bclx_status status;
+  std::string msg;
+};
+
+#define BCLX_HIP(call)                                                                                    \
+  do {                                                                                                     \
+    hipError_t e_ = (call);                                                                                \
+    if (e_ != hipSuccess) {                                                                                \
+      throw ::bclx::Error{e_ == hipErrorOutOfMemory ? BCLX_ERR_ALLOC : BCLX_ERR_HIP,                      \
+                          std::string(#call) + ": " + hipGetErrorString(e_) + " at " + __FILE__ + ":" +   \
+                              std::to_string(__LINE__)};                                                   \
+    }                                                                                                      \
+  } while (0)
+
+#define BCLX_REQUIRE(cond, status, msg)                       \
+  do {                                                        \
+    if (!(cond)) throw ::bclx::Error{(status), (msg)};        \
+  } while (0)
+
+// Device buffer owned by the library (hipMalloc / hipFree). Move-only.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { alloc(n); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t n) {
+    release();
+    if (n == 0) n = 256;
+    BCLX_HIP(hipMalloc(&p, n));
+    bytes = n;
+  }
+  // grow-only
+  void reserve(size_t n) { if (n > bytes) alloc(n); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Bump allocator over one device buffer: sub-buffers are 256-byte aligned.
+struct Arena {
+  DevBuf buf;
+  size_t used = 0;
+  std::vector<std::pair<size_t, size_t>> plan;  // (offset, bytes) of each reservation
+  size_t reserve(size_t bytes) {
+    size_t off = (used + 255) & ~size_t(255);
+    used = off + bytes;
+    return off;
+  }
+  void commit() { buf.reserve(used); }
+  template <typename T> T* at(size_t off) const { return reinterpret_cast<T*>(static_cast<char*>(buf.p) + off); }
+};
+
+}  // namespace bclx
+
+struct bclx_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 256;
+  size_t lds_per_block = 65536;
+  // scratch reused across calls (grow-only)
+  bclx::DevBuf scratch;
+  bclx::DevBuf dense_slots;  // heavy-row dense accumulators, kept zeroed between uses
+  size_t dense_slot_doubles = 0;
+  int dense_nslots = 0;
+  bclx::DevBuf pinned_dummy;
+  int64_t* h_pinned = nullptr;  // small pinned host buffer for counts
+};
+
+namespace bclx {
+
+// Wraps a C-ABI body: converts exceptions to status codes and records the message.
+template <typename F>
+bclx_status guarded(F&& f) {
+  try {
+    f();
+    return BCLX_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return BCLX_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return BCLX_ERR_INVALID;
+  }
+}
+
+inline void set_device(bclx_handle* h) { BCLX_HIP(hipSetDevice(h->device)); }
+
+inline unsigned div_up(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Zeroed heavy-row accumulators: nslots x ndoubles, grown on demand, zeroed once when (re)allocated.
+void ensure_dense_slots(bclx_handle* h, int nslots, size_t ndoubles);
+
+}  // namespace bclx

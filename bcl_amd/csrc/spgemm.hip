@@ -1,0 +1,413 @@
+// SpGEMM host orchestration: the per-product two-phase path (bclx_spgemm_symbolic / bclx_spgemm_numeric) and the
+// flat-row workspace shared with the batched stage pipeline (stage.hip).
+#include "spgemm.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace bclx {
+
+// ---------------------------------------------------------------------------------------------------------------
+// small fill kernels (used instead of hipMemsetAsync so every pipeline can be captured in a HIP graph)
+// ---------------------------------------------------------------------------------------------------------------
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f64(double* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+void fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
+  if (n <= 0) return;
+  unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_i32, dim3(g), dim3(256), 0, s, p, n, v);
+  BCLX_HIP(hipGetLastError());
+}
+void fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v) {
+  if (n <= 0) return;
+  unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u64, dim3(g), dim3(256), 0, s, p, n, v);
+  BCLX_HIP(hipGetLastError());
+}
+
+void ensure_dense_slots(bclx_handle* h, int nslots, size_t ndoubles) {
+  if (h->dense_nslots >= nslots && h->dense_slot_doubles >= ndoubles) return;
+  int ns = std::max(nslots, h->dense_nslots);
+  size_t nd = std::max(ndoubles, h->dense_slot_doubles);
+  BCLX_HIP(hipStreamSynchronize(h->stream));
+  h->dense_slots.alloc((size_t)ns * nd * sizeof(double));
+  int64_t tot = (int64_t)ns * (int64_t)nd;
+  unsigned g = (unsigned)std::min<int64_t>((tot + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_fill_f64, dim3(g), dim3(256), 0, h->stream, h->dense_slots.as<double>(), tot);
+  BCLX_HIP(hipGetLastError());
+  BCLX_HIP(hipStreamSynchronize(h->stream));
+  h->dense_nslots = ns;
+  h->dense_slot_doubles = nd;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// FlatRows
+// ---------------------------------------------------------------------------------------------------------------
+
+template <typename T, typename RP>
+void FlatRows<T, RP>::setup(bclx_handle* h_, const std::vector<TileDev<T, RP>>& tiles_in, bool with_cap) {
+  h = h_;
+  tiles = tiles_in;
+  ntiles = (int)tiles.size();
+  nflat = 0;
+  max_n = 1;
+  std::vector<uint16_t> blk;
+  for (int t = 0; t < ntiles; t++) {
+    tiles[t].flat0 = nflat;
+    int64_t pm = ((int64_t)tiles[t].m + kRowBlock - 1) / kRowBlock * kRowBlock;
+    if (pm == 0) pm = kRowBlock;  // every product owns at least one block
+    for (int64_t b = 0; b < pm / kRowBlock; b++) blk.push_back((uint16_t)t);
+    nflat += pm;
+    max_n = std::max<int64_t>(max_n, tiles[t].n);
+  }
+  BCLX_REQUIRE(ntiles <= 65535, BCLX_ERR_INVALID, "at most 65535 products per batch");
+  BCLX_REQUIRE(nflat < INT32_MAX, BCLX_ERR_OVERFLOW, "too many rows in one batch (flat rows must fit int32)");
+  nblocks = nflat / kRowBlock;
+  Arena& A = arena;
+  A.used = 0;
+  size_t o_tiles = A.reserve(sizeof(TileDev<T, RP>) * ntiles);
+  size_t o_blk = A.reserve(sizeof(uint16_t) * nblocks);
+  size_t o_prod = A.reserve(sizeof(int64_t) * nflat);
+  size_t o_nnz = A.reserve(sizeof(int32_t) * nflat);
+  size_t o_cap = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
+  size_t o_off = A.reserve(sizeof(int64_t) * (nflat + 1));
+  size_t o_bc = A.reserve(sizeof(int32_t) * kNumBins);
+  size_t o_bi = A.reserve(sizeof(int32_t) * kNumBins * nflat);
+  size_t o_oc = A.reserve(sizeof(int32_t) * 2);
+  size_t o_oi = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
+  size_t o_cur = A.reserve(sizeof(unsigned long long) * 2);
+  size_t o_stats = A.reserve(sizeof(unsigned long long) * 8);
+  size_t nsb = (size_t)((nflat + kScanTile - 1) / kScanTile) + 1;
+  size_t o_part = A.reserve(sizeof(long long) * nsb);
+  size_t o_tot = A.reserve(sizeof(int64_t) * 2);
+  A.commit();
+  R.tiles = A.at<TileDev<T, RP>>(o_tiles);
+  R.blk_tile = A.at<uint16_t>(o_blk);
+  R.prod = A.at<int64_t>(o_prod);
+  R.nnz = A.at<int32_t>(o_nnz);
+  R.cap = A.at<int32_t>(o_cap);
+  R.off = A.at<int64_t>(o_off);
+  R.bin_count = A.at<int32_t>(o_bc);
+  R.bin_items = A.at<int32_t>(o_bi);
+  R.bin_stride = nflat;
+  R.ovf_count = A.at<int32_t>(o_oc);
+  R.ovf_items = A.at<int32_t>(o_oi);
+  R.cursor = A.at<unsigned long long>(o_cur);
+  R.stats = A.at<unsigned long long>(o_stats);
+  scan_partial = A.at<long long>(o_part);
+  scan_total = A.at<int64_t>(o_tot);
+  BCLX_HIP(hipMemcpyAsync(const_cast<TileDev<T, RP>*>(R.tiles), tiles.data(), sizeof(TileDev<T, RP>) * ntiles,
+                          hipMemcpyHostToDevice, h->stream));
+  BCLX_HIP(hipMemcpyAsync(const_cast<uint16_t*>(R.blk_tile), blk.data(), sizeof(uint16_t) * nblocks,
+                          hipMemcpyHostToDevice, h->stream));
+  BCLX_HIP(hipStreamSynchronize(h->stream));
+}
+
+template <typename T, typename RP>
+void FlatRows<T, RP>::upload_tiles() {
+  BCLX_HIP(hipMemcpyAsync(const_cast<TileDev<T, RP>*>(R.tiles), tiles.data(), sizeof(TileDev<T, RP>) * ntiles,
+                          hipMemcpyHostToDevice, h->stream));
+}
+
+template <typename T, typename RP>
+void FlatRows<T, RP>::reset_counters(hipStream_t s) {
+  fill_i32(s, R.bin_count, kNumBins, 0);
+  fill_i32(s, R.ovf_count, 2, 0);
+  fill_u64(s, R.stats, 8, 0);
+}
+
+template <typename T, typename RP>
+void FlatRows<T, RP>::analyze(hipStream_t s, bool do_bin) {
+  hipLaunchKernelGGL((k_analyze<T, RP>), dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, R, do_bin ? 1 : 0);
+  BCLX_HIP(hipGetLastError());
+}
+
+template <typename T, typename RP>
+void FlatRows<T, RP>::bin(hipStream_t s, int src) {
+  fill_i32(s, R.bin_count, kNumBins, 0);
+  hipLaunchKernelGGL((k_bin<T, RP>), dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, R, src);
+  BCLX_HIP(hipGetLastError());
+}
+
+template <typename T, typename RP>
+void FlatRows<T, RP>::scan(hipStream_t s, const int32_t* cnt, int64_t* off, int64_t off_add) {
+  const int64_t nb = (nflat + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanBlock), 0, s, cnt, nflat, scan_partial);
+  BCLX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, scan_partial, nb, scan_total);
+  BCLX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kScanBlock), 0, s, cnt, nflat, scan_partial, off, off_add);
+  BCLX_HIP(hipGetLastError());
+}
+
+template <int B, int MODE, typename T, typename RP>
+static void launch_bin_kernel(const RowsDev<T, RP>& R, hipStream_t s, unsigned grid) {
+  constexpr int TS = bin_table(B), G = bin_group(B);
+  constexpr int BLOCK = G < 256 ? 256 : G;
+  hipLaunchKernelGGL((k_hash<TS, G, BLOCK, MODE, T, RP>), dim3(grid), dim3(BLOCK), 0, s, R, B);
+  BCLX_HIP(hipGetLastError());
+}
+
+template <int MODE, typename T, typename RP>
+static void launch_lds_bin(const RowsDev<T, RP>& R, hipStream_t s, int b, unsigned grid) {
+  switch (b) {
+    case 1: launch_bin_kernel<1, MODE>(R, s, grid); break;
+    case 2: launch_bin_kernel<2, MODE>(R, s, grid); break;
+    case 3: launch_bin_kernel<3, MODE>(R, s, grid); break;
+    case 4: launch_bin_kernel<4, MODE>(R, s, grid); break;
+    case 5: launch_bin_kernel<5, MODE>(R, s, grid); break;
+    case 6: launch_bin_kernel<6, MODE>(R, s, grid); break;
+    case 7: launch_bin_kernel<7, MODE>(R, s, grid); break;
+    case 8: launch_bin_kernel<8, MODE>(R, s, grid); break;
+    case 9: launch_bin_kernel<9, MODE>(R, s, grid); break;
+    default: throw Error{BCLX_ERR_INVALID, "bad LDS bin"};
+  }
+}
+
+int rows_per_block(int b) {
+  const int G = bin_group(b);
+  const int BLOCK = G < 256 ? 256 : G;
+  return BLOCK / G;
+}
+
+template <typename T, typename RP>
+void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint) {
+  // count_hint >= 0: exact row count known on the host (grid sized to it); < 0: persistent grid.
+  if (b == 0 || count_hint == 0) return;
+  if (b == kHeavyBin) {
+    const int64_t words = (max_n + 31) / 32;
+    const bool lds_bitmap = words * 4 <= (int64_t)h->lds_per_block - 4096;
+    int nslots = (int)std::min<int64_t>(count_hint >= 0 ? count_hint : h->num_cus, (int64_t)h->num_cus);
+    if (nslots < 1) nslots = 1;
+    ensure_dense_slots(h, nslots, (size_t)max_n);
+    if (!lds_bitmap) gbits.reserve(sizeof(unsigned) * (size_t)words * nslots + 256);
+    // zeroed global bitmaps are kept zero by the kernel; zero them once on (re)allocation
+    if (!lds_bitmap && !gbits_zeroed) {
+      fill_i32(s, gbits.as<int32_t>(), (int64_t)(gbits.bytes / 4), 0);
+      gbits_zeroed = true;
+    }
+    const size_t shm = lds_bitmap ? (size_t)words * 4 : 0;
+    double* dense = h->dense_slots.as<double>();
+    const int64_t sd = (int64_t)h->dense_slot_doubles;
+    unsigned* gb = lds_bitmap ? nullptr : gbits.as<unsigned>();
+#define BCLX_HEAVY(M, LB)                                                                                 \
+  do {                                                                                                    \
+    auto kfn = k_heavy<M, LB, T, RP>;                                                                     \
+    if (shm > 0) BCLX_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)); \
+    hipLaunchKernelGGL(kfn, dim3(nslots), dim3(kHeavyThreads), shm, s, R, kHeavyBin, dense, sd, gb, words);  \
+  } while (0)
+    if (lds_bitmap) {
+      switch (mode) { case 0: BCLX_HEAVY(0, true); break; case 1: BCLX_HEAVY(1, true); break;
+                      case 2: BCLX_HEAVY(2, true); break; default: BCLX_HEAVY(3, true); break; }
+    } else {
+      switch (mode) { case 0: BCLX_HEAVY(0, false); break; case 1: BCLX_HEAVY(1, false); break;
+                      case 2: BCLX_HEAVY(2, false); break; default: BCLX_HEAVY(3, false); break; }
+    }
+#undef BCLX_HEAVY
+    BCLX_HIP(hipGetLastError());
+    return;
+  }
+  const int rpb = rows_per_block(b);
+  const int64_t per_cu = 8;  // resident blocks per CU targeted by the persistent grid
+  unsigned grid;
+  if (count_hint >= 0) grid = (unsigned)std::max<int64_t>(1, (count_hint + rpb - 1) / rpb);
+  else grid = (unsigned)(h->num_cus * per_cu);
+  grid = std::min<unsigned>(grid, 1u << 20);
+  switch (mode) {
+    case 0: launch_lds_bin<0>(R, s, b, grid); break;
+    case 1: launch_lds_bin<1>(R, s, b, grid); break;
+    case 2: launch_lds_bin<2>(R, s, b, grid); break;
+    default: launch_lds_bin<3>(R, s, b, grid); break;
+  }
+}
+
+template <typename T, typename RP>
+void FlatRows<T, RP>::compact(hipStream_t s, const int64_t* dst_off, int* dst_col, T* dst_val, const int* src_col,
+                              const T* src_val) {
+  hipLaunchKernelGGL((k_compact<T, RP>), dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, R, dst_off, dst_col, dst_val,
+                     src_col, src_val);
+  BCLX_HIP(hipGetLastError());
+}
+
+template struct FlatRows<float, int32_t>;
+template struct FlatRows<float, int64_t>;
+template struct FlatRows<double, int32_t>;
+template struct FlatRows<double, int64_t>;
+
+// ---------------------------------------------------------------------------------------------------------------
+// two-phase per-product path
+// ---------------------------------------------------------------------------------------------------------------
+
+static void check_csr(const bclx_csr* x, const char* name) {
+  BCLX_REQUIRE(x != nullptr, BCLX_ERR_INVALID, std::string(name) + " is null");
+  BCLX_REQUIRE(x->m >= 0 && x->n >= 0 && x->nnz >= 0, BCLX_ERR_INVALID, std::string(name) + ": negative size");
+  BCLX_REQUIRE(x->ptr_width == 4 || x->ptr_width == 8, BCLX_ERR_INVALID, std::string(name) + ": ptr_width must be 4 or 8");
+  BCLX_REQUIRE(x->dtype == BCLX_F32 || x->dtype == BCLX_F64, BCLX_ERR_INVALID,
+               std::string(name) + ": dtype must be BCLX_F32 or BCLX_F64");
+  BCLX_REQUIRE(x->row_ptr != nullptr, BCLX_ERR_INVALID, std::string(name) + ": row_ptr is null");
+  BCLX_REQUIRE(x->nnz == 0 || (x->col_idx && x->vals), BCLX_ERR_INVALID, std::string(name) + ": null arrays");
+  BCLX_REQUIRE(x->m < INT32_MAX && x->n < INT32_MAX, BCLX_ERR_INVALID, std::string(name) + ": dimension >= 2^31");
+  BCLX_REQUIRE(x->ptr_width == 8 || x->nnz <= INT32_MAX, BCLX_ERR_OVERFLOW, std::string(name) + ": nnz > int32 row_ptr");
+}
+
+void check_pair(const bclx_csr* A, const bclx_csr* B) {
+  check_csr(A, "A");
+  check_csr(B, "B");
+  BCLX_REQUIRE(A->n == B->m, BCLX_ERR_INVALID,
+               "inner dimensions differ: A is " + std::to_string(A->m) + "x" + std::to_string(A->n) + ", B is " +
+                   std::to_string(B->m) + "x" + std::to_string(B->n));
+  BCLX_REQUIRE(A->dtype == B->dtype, BCLX_ERR_INVALID, "A and B value types differ");
+  BCLX_REQUIRE(A->ptr_width == B->ptr_width, BCLX_ERR_INVALID, "A and B row_ptr widths differ");
+}
+
+template <typename T, typename RP>
+TileDev<T, RP> make_tile(const bclx_csr* A, const bclx_csr* B) {
+  TileDev<T, RP> t{};
+  t.a_rp = static_cast<const RP*>(A->row_ptr);
+  t.a_col = A->col_idx;
+  t.a_val = static_cast<const T*>(A->vals);
+  t.b_rp = static_cast<const RP*>(B->row_ptr);
+  t.b_col = B->col_idx;
+  t.b_val = static_cast<const T*>(B->vals);
+  t.m = (int32_t)A->m;
+  t.n = (int32_t)B->n;
+  return t;
+}
+template TileDev<float, int32_t> make_tile<float, int32_t>(const bclx_csr*, const bclx_csr*);
+template TileDev<float, int64_t> make_tile<float, int64_t>(const bclx_csr*, const bclx_csr*);
+template TileDev<double, int32_t> make_tile<double, int32_t>(const bclx_csr*, const bclx_csr*);
+template TileDev<double, int64_t> make_tile<double, int64_t>(const bclx_csr*, const bclx_csr*);
+
+struct PlanBase {
+  virtual ~PlanBase() = default;
+  virtual int64_t symbolic() = 0;
+  virtual void numeric(bclx_csr* C) = 0;
+  virtual void rows(int64_t* prod, int32_t* nnz) = 0;
+};
+
+template <typename T, typename RP>
+struct TwoPhase : PlanBase {
+  bclx_handle* h;
+  FlatRows<T, RP> F;
+  int64_t nnz_c = -1;
+  int64_t m = 0;
+  TwoPhase(bclx_handle* h_, const bclx_csr* A, const bclx_csr* B) : h(h_) {
+    std::vector<TileDev<T, RP>> tiles{make_tile<T, RP>(A, B)};
+    F.setup(h, tiles, false);
+    m = A->m;
+  }
+  int64_t symbolic() override {
+    hipStream_t s = h->stream;
+    F.reset_counters(s);
+    F.analyze(s, true);
+    int32_t counts[kNumBins];
+    BCLX_HIP(hipMemcpyAsync(counts, F.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
+    BCLX_HIP(hipStreamSynchronize(s));
+    for (int b = 1; b < kNumBins; b++) F.run_bin(s, 0, b, counts[b]);
+    F.scan(s, F.R.nnz, F.R.off, 0);
+    int64_t tot = 0;
+    BCLX_HIP(hipMemcpyAsync(&tot, F.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    BCLX_HIP(hipStreamSynchronize(s));
+    nnz_c = tot;
+    return tot;
+  }
+  void numeric(bclx_csr* C) override {
+    BCLX_REQUIRE(nnz_c >= 0, BCLX_ERR_INVALID, "bclx_spgemm_numeric before bclx_spgemm_symbolic");
+    BCLX_REQUIRE(C && C->row_ptr, BCLX_ERR_INVALID, "C or C->row_ptr is null");
+    BCLX_REQUIRE(C->m == m, BCLX_ERR_INVALID, "C has the wrong number of rows");
+    BCLX_REQUIRE(C->nnz >= nnz_c, BCLX_ERR_INVALID, "C arrays smaller than nnz from the symbolic phase");
+    BCLX_REQUIRE(nnz_c == 0 || (C->col_idx && C->vals), BCLX_ERR_INVALID, "C->col_idx / C->vals are null");
+    BCLX_REQUIRE(C->ptr_width == 8 || nnz_c <= INT32_MAX, BCLX_ERR_OVERFLOW,
+                 "nnz(C) = " + std::to_string(nnz_c) + " does not fit an int32 row_ptr; request ptr_width 8");
+    BCLX_REQUIRE(C->dtype == (std::is_same<T, float>::value ? BCLX_F32 : BCLX_F64), BCLX_ERR_INVALID,
+                 "C dtype must match A and B");
+    hipStream_t s = h->stream;
+    F.tiles[0].c_col = C->col_idx;
+    F.tiles[0].c_val = static_cast<T*>(C->vals);
+    F.upload_tiles();
+    F.bin(s, 0);
+    int32_t counts[kNumBins];
+    BCLX_HIP(hipMemcpyAsync(counts, F.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
+    BCLX_HIP(hipStreamSynchronize(s));
+    for (int b = 1; b < kNumBins; b++) F.run_bin(s, 1, b, counts[b]);
+    const unsigned g = div_up(m + 1, 256);
+    if (C->ptr_width == 8)
+      hipLaunchKernelGGL(k_rowptr<int64_t>, dim3(g), dim3(256), 0, s, F.R.off, (int64_t)0, m,
+                         static_cast<int64_t*>(C->row_ptr));
+    else
+      hipLaunchKernelGGL(k_rowptr<int32_t>, dim3(g), dim3(256), 0, s, F.R.off, (int64_t)0, m,
+                         static_cast<int32_t*>(C->row_ptr));
+    BCLX_HIP(hipGetLastError());
+    C->nnz = nnz_c;
+  }
+  void rows(int64_t* prod, int32_t* nnz) override {
+    if (prod) BCLX_HIP(hipMemcpy(prod, F.R.prod, sizeof(int64_t) * m, hipMemcpyDeviceToHost));
+    if (nnz) BCLX_HIP(hipMemcpy(nnz, F.R.nnz, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+  }
+};
+
+}  // namespace bclx
+
+struct bclx_spgemm_plan {
+  bclx_handle* h;
+  std::unique_ptr<bclx::PlanBase> impl;
+};
+
+using namespace bclx;
+
+extern "C" {
+
+bclx_status bclx_spgemm_symbolic(bclx_handle* h, const bclx_csr* A, const bclx_csr* B, bclx_spgemm_plan** plan,
+                                 int64_t* nnz_c) {
+  return guarded([&] {
+    BCLX_REQUIRE(h && plan && nnz_c, BCLX_ERR_INVALID, "null handle / plan / nnz_c");
+    check_pair(A, B);
+    set_device(h);
+    auto p = std::make_unique<bclx_spgemm_plan>();
+    p->h = h;
+    const bool f64 = A->dtype == BCLX_F64;
+    const bool w8 = A->ptr_width == 8;
+    if (!f64 && !w8) p->impl = std::make_unique<TwoPhase<float, int32_t>>(h, A, B);
+    else if (!f64 && w8) p->impl = std::make_unique<TwoPhase<float, int64_t>>(h, A, B);
+    else if (f64 && !w8) p->impl = std::make_unique<TwoPhase<double, int32_t>>(h, A, B);
+    else p->impl = std::make_unique<TwoPhase<double, int64_t>>(h, A, B);
+    *nnz_c = p->impl->symbolic();
+    *plan = p.release();
+  });
+}
+
+bclx_status bclx_spgemm_numeric(bclx_spgemm_plan* plan, bclx_csr* C) {
+  return guarded([&] {
+    BCLX_REQUIRE(plan, BCLX_ERR_INVALID, "null plan");
+    set_device(plan->h);
+    plan->impl->numeric(C);
+  });
+}
+
+bclx_status bclx_spgemm_plan_rows(bclx_spgemm_plan* plan, int64_t* h_products, int32_t* h_row_nnz) {
+  return guarded([&] {
+    BCLX_REQUIRE(plan, BCLX_ERR_INVALID, "null plan");
+    set_device(plan->h);
+    BCLX_HIP(hipStreamSynchronize(plan->h->stream));
+    plan->impl->rows(h_products, h_row_nnz);
+  });
+}
+
+bclx_status bclx_spgemm_plan_destroy(bclx_spgemm_plan* plan) {
+  return guarded([&] {
+    if (!plan) return;
+    set_device(plan->h);
+    (void)hipStreamSynchronize(plan->h->stream);
+    delete plan;
+  });
+}
+
+}  // extern "C"

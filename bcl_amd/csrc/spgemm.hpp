@@ -1,0 +1,45 @@
+// SpGEMM host-side building blocks shared by the two-phase path and the batched stage pipeline.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "internal.hpp"
+#include "spgemm_kernels.hpp"
+
+namespace bclx {
+
+void fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+void fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v);
+int rows_per_block(int bin);
+void check_pair(const bclx_csr* A, const bclx_csr* B);
+
+template <typename T, typename RP>
+TileDev<T, RP> make_tile(const bclx_csr* A, const bclx_csr* B);
+
+// Device workspace over the flat-row space of one batch of products.
+template <typename T, typename RP>
+struct FlatRows {
+  bclx_handle* h = nullptr;
+  std::vector<TileDev<T, RP>> tiles;
+  int ntiles = 0;
+  int64_t nflat = 0, nblocks = 0, max_n = 1;
+  Arena arena;
+  RowsDev<T, RP> R{};
+  long long* scan_partial = nullptr;
+  int64_t* scan_total = nullptr;
+  DevBuf gbits;
+  bool gbits_zeroed = false;
+
+  void setup(bclx_handle* h, const std::vector<TileDev<T, RP>>& tiles, bool with_cap);
+  void upload_tiles();
+  void reset_counters(hipStream_t s);
+  void analyze(hipStream_t s, bool do_bin);
+  void bin(hipStream_t s, int src);
+  // exclusive scan of cnt over all flat rows into off (nflat + 1 entries), plus off_add; total -> *scan_total
+  void scan(hipStream_t s, const int32_t* cnt, int64_t* off, int64_t off_add);
+  void run_bin(hipStream_t s, int mode, int b, int64_t count_hint);
+  void compact(hipStream_t s, const int64_t* dst_off, int* dst_col, T* dst_val, const int* src_col, const T* src_val);
+};
+
+}  // namespace bclx

@@ -1,0 +1,552 @@
+// SpGEMM device kernels: row analysis, binning, LDS-hash accumulators (symbolic / numeric / predicted / one-pass),
+// heavy-row global accumulators, scans and compaction. Shared by the per-product two-phase path and the batched
+// stage pipeline: every kernel works on a "flat row" space that concatenates the rows of all products of a batch,
+// each product padded to a multiple of 256 rows, so one 256-row block never straddles two products.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+namespace bclx {
+
+constexpr int kEmptyKey = 0x7fffffff;  // larger than any column index
+constexpr int kNumBins = 11;           // 0: empty row; 1..9: LDS tables; 10: heavy (global accumulator)
+constexpr int kHeavyBin = 10;
+constexpr int kRowBlock = 256;         // flat-row padding and analysis block size
+constexpr int kHeavyThreads = 1024;
+
+// Table geometry of LDS bins 1..9: TS = 2^(b+4) slots holds up to TS/2 keys (load <= 1/2); G lanes per row.
+__host__ __device__ constexpr int bin_table(int b) { return 1 << (b + 4); }
+__host__ __device__ constexpr int bin_group(int b) {
+  return b == 1 ? 16 : b == 2 ? 32 : b <= 4 ? 64 : b == 5 ? 128 : b <= 7 ? 256 : b == 8 ? 512 : 1024;
+}
+
+// Tunable binning knobs (chosen by the autotuner; these are the defaults).
+struct BinTuning {
+  int32_t work_per_lane = 64;     // a row's group must have >= products / work_per_lane lanes
+  int32_t heavy_need = 4096;      // rows needing more table entries than this use the global accumulator
+  int32_t load_shift = 1;         // table slots >= need << load_shift (1: load <= 1/2)
+};
+
+__host__ __device__ inline int bin_for(int64_t need, int64_t work, const BinTuning& tn) {
+  if (work <= 0) return 0;
+  if (need > tn.heavy_need) return kHeavyBin;
+  int64_t slots = need << tn.load_shift;
+  int bn = 1;
+  while (bn < 9 && bin_table(bn) < slots) bn++;
+  if (bin_table(bn) < slots) return kHeavyBin;
+  int bw = bn;
+  while (bw < 9 && (int64_t)bin_group(bw) * tn.work_per_lane < work) bw++;
+  return bw;
+}
+
+template <typename T, typename RP>
+struct TileDev {
+  const RP* a_rp;
+  const int* a_col;
+  const T* a_val;
+  const RP* b_rp;
+  const int* b_col;
+  const T* b_val;
+  int* c_col;  // output base (positions come from off[])
+  T* c_val;
+  int64_t flat0;  // first flat row of this product
+  int32_t m, n;
+};
+
+template <typename T, typename RP>
+struct RowsDev {
+  const TileDev<T, RP>* tiles;
+  const uint16_t* blk_tile;  // (flat row >> 8) -> product index
+  int64_t* prod;             // products per flat row
+  int32_t* nnz;              // nnz(C_i): symbolic count, or measured count
+  int32_t* cap;              // predicted capacity (learned path)
+  int64_t* off;              // output offset per flat row (nflat + 1 entries)
+  int32_t* bin_count;        // kNumBins counters
+  int32_t* bin_items;        // kNumBins lists of flat rows, bin_stride apart
+  int64_t bin_stride;
+  int32_t* ovf_count;        // overflow list (learned path)
+  int32_t* ovf_items;
+  unsigned long long* cursor;  // bump allocator for rows placed outside their predicted slot
+  unsigned long long* stats;   // [0] products [1] heavy rows [2] overflow rows
+  BinTuning tn;
+};
+
+// ------------------------------------------------------------------------------------------------------------
+// reductions / scans
+// ------------------------------------------------------------------------------------------------------------
+
+template <typename V>
+__device__ inline V wave_sum(V v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Inclusive scan within a wave.
+template <typename V>
+__device__ inline V wave_incl_scan(V v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    V t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan (BLOCK a multiple of 64, <= 1024). Returns exclusive prefix; *total = block sum.
+template <int BLOCK, typename V>
+__device__ inline V block_excl_scan(V v, V* total, V* smem /* >= BLOCK/64 + 1 */) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int NW = BLOCK / 64;
+  V inc = wave_incl_scan(v);
+  if (lane == 63) smem[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    V run = 0;
+    for (int q = 0; q < NW; q++) {
+      V t = smem[q];
+      smem[q] = run;
+      run += t;
+    }
+    smem[NW] = run;
+  }
+  __syncthreads();
+  V r = smem[w] + inc - v;
+  *total = smem[NW];
+  __syncthreads();
+  return r;
+}
+
+template <int BLOCK, typename V>
+__device__ inline V block_sum(V v, V* smem) {
+  V t;
+  (void)block_excl_scan<BLOCK, V>(v, &t, smem);
+  return t;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// row analysis: products per row, optional binning by upper bound min(products, n) (symbolic bins)
+// ------------------------------------------------------------------------------------------------------------
+
+template <typename T, typename RP>
+__global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_bin) {
+  __shared__ int s_long[kRowBlock];
+  __shared__ int s_nlong;
+  __shared__ int s_bcount[kNumBins], s_bbase[kNumBins];
+  __shared__ long long s_red[kRowBlock / 64 + 1];
+  const int tid = threadIdx.x;
+  const int64_t fr = (int64_t)blockIdx.x * kRowBlock + tid;
+  const TileDev<T, RP> tile = R.tiles[R.blk_tile[blockIdx.x]];
+  const int64_t i = fr - tile.flat0;
+  const bool valid = i >= 0 && i < tile.m;
+  if (tid == 0) s_nlong = 0;
+  if (tid < kNumBins) s_bcount[tid] = 0;
+  __syncthreads();
+  long long prod = 0;
+  if (valid) {
+    RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
+    if (a1 - a0 <= 32) {
+      for (RP p = a0; p < a1; p++) {
+        int k = tile.a_col[p];
+        prod += (long long)(tile.b_rp[k + 1] - tile.b_rp[k]);
+      }
+    } else {
+      s_long[atomicAdd(&s_nlong, 1)] = tid;
+    }
+  }
+  __syncthreads();
+  const int nlong = s_nlong;
+  for (int q = 0; q < nlong; q++) {
+    const int lt = s_long[q];
+    const int64_t li = (int64_t)blockIdx.x * kRowBlock + lt - tile.flat0;
+    RP a0 = tile.a_rp[li], a1 = tile.a_rp[li + 1];
+    long long s = 0;
+    for (RP p = a0 + tid; p < a1; p += kRowBlock) {
+      int k = tile.a_col[p];
+      s += (long long)(tile.b_rp[k + 1] - tile.b_rp[k]);
+    }
+    s = block_sum<kRowBlock, long long>(s, s_red);
+    if (tid == lt) prod = s;
+  }
+  R.prod[fr] = valid ? prod : 0;
+  if (do_bin) {
+    const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
+    const int b = valid ? bin_for(ub, prod, R.tn) : 0;
+    int pos = 0;
+    if (b > 0) pos = atomicAdd(&s_bcount[b], 1);
+    __syncthreads();
+    if (tid > 0 && tid < kNumBins && s_bcount[tid] > 0) s_bbase[tid] = atomicAdd(&R.bin_count[tid], s_bcount[tid]);
+    __syncthreads();
+    if (b > 0) R.bin_items[b * R.bin_stride + s_bbase[b] + pos] = (int)fr;
+    if (b == 0) R.nnz[fr] = 0;  // empty and padding rows
+  }
+  long long tot = block_sum<kRowBlock, long long>(prod, s_red);
+  if (tid == 0 && tot) atomicAdd(&R.stats[0], (unsigned long long)tot);
+}
+
+// Bin rows by `need` (src 0: nnz, 1: cap) and products. Padded / empty rows go nowhere.
+template <typename T, typename RP>
+__global__ __launch_bounds__(kRowBlock) void k_bin(RowsDev<T, RP> R, int src) {
+  __shared__ int s_bcount[kNumBins], s_bbase[kNumBins];
+  const int tid = threadIdx.x;
+  const int64_t fr = (int64_t)blockIdx.x * kRowBlock + tid;
+  const TileDev<T, RP> tile = R.tiles[R.blk_tile[blockIdx.x]];
+  const int64_t i = fr - tile.flat0;
+  const bool valid = i >= 0 && i < tile.m;
+  if (tid < kNumBins) s_bcount[tid] = 0;
+  __syncthreads();
+  int b = 0;
+  if (valid) {
+    const long long work = R.prod[fr];
+    const long long need = src == 0 ? (long long)R.nnz[fr] : (long long)R.cap[fr];
+    b = (need == 0 && src == 0) ? 0 : bin_for(need, work, R.tn);
+  }
+  int pos = 0;
+  if (b > 0) pos = atomicAdd(&s_bcount[b], 1);
+  __syncthreads();
+  if (tid > 0 && tid < kNumBins && s_bcount[tid] > 0) s_bbase[tid] = atomicAdd(&R.bin_count[tid], s_bcount[tid]);
+  __syncthreads();
+  if (b > 0) R.bin_items[b * R.bin_stride + s_bbase[b] + pos] = (int)fr;
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// LDS hash accumulators
+//   MODE 0 symbolic: count distinct columns -> nnz[fr]
+//   MODE 1 numeric: accumulate, sort, write at off[fr] (nnz known)
+//   MODE 2 predicted: table sized by cap[fr]; more than cap distinct columns -> overflow list, else write at off[fr]
+//   MODE 3 one-pass exact: table sized by the upper bound; count, bump-allocate from *cursor, write
+// Values accumulate in float64 (products of fp32 inputs are exact in float64) and round once on output.
+// ------------------------------------------------------------------------------------------------------------
+
+template <int TS, int G, int BLOCK>
+__device__ inline void bitonic_sort_groups(int* keys, double* vals, int lane, bool with_vals) {
+  // Sort each group's TS-slot table ascending by key; kEmptyKey sorts last. All threads of the block call this.
+  for (int k = 2; k <= TS; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = lane; t < TS / 2; t += G) {
+        const int i = 2 * t - (t & (j - 1));
+        const int p = i + j;
+        const bool asc = (i & k) == 0;
+        const int ki = keys[i], kp = keys[p];
+        if ((ki > kp) == asc) {
+          keys[i] = kp;
+          keys[p] = ki;
+          if (with_vals) {
+            double vi = vals[i];
+            vals[i] = vals[p];
+            vals[p] = vi;
+          }
+        }
+      }
+      if (G > 64) {
+        __syncthreads();
+      } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      }
+    }
+  }
+}
+
+template <int TS, int G, int BLOCK, int MODE, typename T, typename RP>
+__global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
+  constexpr int NG = BLOCK / G;
+  constexpr bool NUM = MODE != 0;
+  __shared__ int s_keys[NG * TS];
+  __shared__ double s_vals[NUM ? NG * TS : 1];
+  __shared__ int s_cnt[NG];
+  __shared__ int s_flag[NG];
+  __shared__ long long s_base[NG];
+  const int g = threadIdx.x / G, lane = threadIdx.x % G;
+  int* keys = s_keys + g * TS;
+  double* vals = s_vals + (NUM ? g * TS : 0);
+  const int count = R.bin_count[bin];
+  const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
+
+  for (int base = blockIdx.x * NG; base < count; base += gridDim.x * NG) {
+    const int idx = base + g;
+    const bool active = idx < count;
+    const int fr = active ? items[idx] : 0;
+    for (int s = lane; s < TS; s += G) {
+      keys[s] = kEmptyKey;
+      if (NUM) vals[s] = 0.0;
+    }
+    if (lane == 0) {
+      s_cnt[g] = 0;
+      s_flag[g] = 0;
+    }
+    __syncthreads();
+    TileDev<T, RP> tile;
+    if (active) {
+      tile = R.tiles[R.blk_tile[fr >> 8]];
+      const int64_t i = fr - tile.flat0;
+      const int limit = MODE == 2 ? R.cap[fr] : INT_MAX;
+      const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
+      const long long prod = R.prod[fr];
+      const long long na = (long long)(a1 - a0);
+      const long long avg = (prod + na - 1) / (na > 0 ? na : 1);
+      int lg = 0;
+      while ((1 << lg) < avg && (1 << lg) < G) lg++;
+      const int L = 1 << lg;
+      const int S = G >> lg;
+      const int sub = lane >> lg, sl = lane & (L - 1);
+      for (RP p = a0 + sub; p < a1; p += S) {
+        if (MODE == 2 && s_flag[g]) break;
+        const int kk = tile.a_col[p];
+        const double av = NUM ? (double)tile.a_val[p] : 0.0;
+        const RP b0 = tile.b_rp[kk], b1 = tile.b_rp[kk + 1];
+        for (RP q = b0 + sl; q < b1; q += L) {
+          const int j = tile.b_col[q];
+          const double pv = NUM ? av * (double)tile.b_val[q] : 0.0;
+          unsigned h = ((unsigned)j * 107u) & (TS - 1);
+          for (int probe = 0; probe < TS; probe++) {
+            const int kcur = keys[h];
+            if (kcur == j) {
+              if (NUM) atomicAdd(&vals[h], pv);
+              break;
+            }
+            if (kcur == kEmptyKey) {
+              const int old = atomicCAS(&keys[h], kEmptyKey, j);
+              if (old == kEmptyKey) {
+                const int c = atomicAdd(&s_cnt[g], 1);
+                if (MODE == 2 && c >= limit) s_flag[g] = 1;
+                if (NUM) atomicAdd(&vals[h], pv);
+                break;
+              }
+              if (old == j) {
+                if (NUM) atomicAdd(&vals[h], pv);
+                break;
+              }
+            }
+            h = (h + 1) & (TS - 1);
+            if (MODE == 2 && probe == TS - 1) s_flag[g] = 1;  // table full: only possible after overflow
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (MODE == 0) {
+      if (active && lane == 0) R.nnz[fr] = s_cnt[g];
+    } else {
+      const int cnt = s_cnt[g];
+      const bool ovf = MODE == 2 && s_flag[g];
+      if (active && lane == 0) {
+        if (MODE == 2) {
+          if (ovf) {
+            int slot = atomicAdd(R.ovf_count, 1);
+            R.ovf_items[slot] = fr;
+            atomicAdd(&R.stats[2], 1ull);
+          } else {
+            R.nnz[fr] = cnt;
+          }
+        } else if (MODE == 3) {
+          s_base[g] = (long long)atomicAdd(R.cursor, (unsigned long long)cnt);
+          R.off[fr] = s_base[g];
+          R.nnz[fr] = cnt;
+        }
+      }
+      bitonic_sort_groups<TS, G, BLOCK>(keys, vals, lane, true);
+      __syncthreads();
+      if (active && !ovf) {
+        const long long o = MODE == 3 ? s_base[g] : R.off[fr];
+        for (int s = lane; s < cnt; s += G) {
+          tile.c_col[o + s] = keys[s];
+          tile.c_val[o + s] = (T)vals[s];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Heavy rows: one workgroup per row, a bitmap of the row's columns (LDS when it fits, else global) and a dense
+// float64 accumulator of n entries in HBM (one zeroed slot per resident workgroup; emission re-zeroes it).
+// The bitmap gives the sorted column order and the exact count without a sort.
+//   MODE 0 symbolic -> nnz[fr]; MODE 1 numeric at off[fr];
+//   MODE 2 predicted: write at off[fr] when count <= cap[fr], else bump-allocate; MODE 3 bump-allocate.
+// ------------------------------------------------------------------------------------------------------------
+
+template <int MODE, bool LDS_BITMAP, typename T, typename RP>
+__global__ __launch_bounds__(kHeavyThreads) void k_heavy(RowsDev<T, RP> R, int bin, double* dense_slots,
+                                                         int64_t slot_doubles, unsigned* gbits, int64_t words) {
+  extern __shared__ unsigned s_dyn[];
+  __shared__ long long s_scan[kHeavyThreads / 64 + 1];
+  __shared__ long long s_base;
+  constexpr bool NUM = MODE != 0;
+  const int tid = threadIdx.x;
+  double* dense = dense_slots + (int64_t)blockIdx.x * slot_doubles;
+  unsigned* bits = LDS_BITMAP ? s_dyn : gbits + (int64_t)blockIdx.x * words;
+  const int count = R.bin_count[bin];
+  const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
+  for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
+    const int fr = items[idx];
+    const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
+    const int64_t i = fr - tile.flat0;
+    const int64_t W = ((int64_t)tile.n + 31) / 32;
+    if (LDS_BITMAP)
+      for (int64_t w = tid; w < W; w += kHeavyThreads) bits[w] = 0u;
+    __syncthreads();
+    const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
+    const long long prod = R.prod[fr];
+    const long long na = (long long)(a1 - a0);
+    const long long avg = (prod + na - 1) / (na > 0 ? na : 1);
+    int lg = 0;
+    while ((1 << lg) < avg && (1 << lg) < kHeavyThreads) lg++;
+    const int L = 1 << lg, S = kHeavyThreads >> lg;
+    const int sub = tid >> lg, sl = tid & (L - 1);
+    for (RP p = a0 + sub; p < a1; p += S) {
+      const int kk = tile.a_col[p];
+      const double av = NUM ? (double)tile.a_val[p] : 0.0;
+      const RP b0 = tile.b_rp[kk], b1 = tile.b_rp[kk + 1];
+      for (RP q = b0 + sl; q < b1; q += L) {
+        const int j = tile.b_col[q];
+        atomicOr(&bits[j >> 5], 1u << (j & 31));
+        if (NUM) unsafeAtomicAdd(&dense[j], av * (double)tile.b_val[q]);
+      }
+    }
+    __threadfence_block();
+    if (!LDS_BITMAP || NUM) __threadfence();
+    __syncthreads();
+    const int64_t per = (W + kHeavyThreads - 1) / kHeavyThreads;
+    const int64_t w0 = tid * per, w1 = (w0 + per < W) ? w0 + per : W;
+    long long c = 0;
+    for (int64_t w = w0; w < w1; w++) c += __popc(bits[w]);
+    long long total;
+    const long long pre = block_excl_scan<kHeavyThreads, long long>(c, &total, s_scan);
+    if (MODE == 0) {
+      if (tid == 0) R.nnz[fr] = (int)total;
+      if (!LDS_BITMAP)
+        for (int64_t w = w0; w < w1; w++) bits[w] = 0u;
+    } else {
+      if (tid == 0) {
+        long long b;
+        if (MODE == 1) {
+          b = R.off[fr];
+        } else if (MODE == 2 && total <= (long long)R.cap[fr]) {
+          b = R.off[fr];
+        } else {
+          b = (long long)atomicAdd(R.cursor, (unsigned long long)total);
+          R.off[fr] = b;
+        }
+        s_base = b;
+        R.nnz[fr] = (int)total;
+      }
+      __syncthreads();
+      long long pos = s_base + pre;
+      for (int64_t w = w0; w < w1; w++) {
+        unsigned word = bits[w];
+        while (word) {
+          const int bit = __ffs(word) - 1;
+          const int j = (int)(w * 32 + bit);
+          tile.c_col[pos] = j;
+          tile.c_val[pos] = (T)dense[j];
+          dense[j] = 0.0;
+          pos++;
+          word &= word - 1;
+        }
+        if (!LDS_BITMAP) bits[w] = 0u;
+      }
+    }
+    if (tid == 0) atomicAdd(&R.stats[1], 1ull);
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Scans over flat rows: off[r] = sum_{q < r} cnt[q] (int32 counts -> int64 offsets), off[n] = total.
+// Three launches: per-block sums, scan of block sums (one block), per-block scan with base.
+// ------------------------------------------------------------------------------------------------------------
+
+constexpr int kScanBlock = 1024;
+constexpr int kScanPer = 4;
+constexpr int kScanTile = kScanBlock * kScanPer;
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int32_t* cnt, int64_t n, long long* partial) {
+  __shared__ long long sm[kScanBlock / 64 + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  long long s = 0;
+  for (int q = 0; q < kScanPer; q++) {
+    const int64_t r = base + (int64_t)q * kScanBlock + threadIdx.x;
+    if (r < n) s += cnt[r];
+  }
+  s = block_sum<kScanBlock, long long>(s, sm);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_top(long long* partial, int64_t nblocks, int64_t* total_out) {
+  __shared__ long long sm[kScanBlock / 64 + 1];
+  long long run = 0;
+  for (int64_t base = 0; base < nblocks; base += kScanBlock) {
+    const int64_t r = base + threadIdx.x;
+    long long v = r < nblocks ? partial[r] : 0;
+    long long tot;
+    long long ex = block_excl_scan<kScanBlock, long long>(v, &tot, sm);
+    if (r < nblocks) partial[r] = run + ex;
+    run += tot;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = run;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_final(const int32_t* cnt, int64_t n, const long long* partial,
+                                                           int64_t* off, int64_t off_add) {
+  __shared__ long long sm[kScanBlock / 64 + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  // each thread owns kScanPer consecutive elements
+  const int64_t r0 = base + (int64_t)threadIdx.x * kScanPer;
+  long long v[kScanPer];
+  long long s = 0;
+  for (int q = 0; q < kScanPer; q++) {
+    v[q] = (r0 + q < n) ? cnt[r0 + q] : 0;
+    s += v[q];
+  }
+  long long tot;
+  long long ex = block_excl_scan<kScanBlock, long long>(s, &tot, sm);
+  long long run = partial[blockIdx.x] + ex + off_add;
+  for (int q = 0; q < kScanPer; q++) {
+    if (r0 + q < n) off[r0 + q] = run;
+    run += v[q];
+  }
+  if (r0 + kScanPer >= n && r0 < n) off[n] = run;  // the thread holding the last element writes the total
+}
+
+// Copy each row's entries from the arena (at src_off[fr], nnz[fr] long) to the final CSR (at dst_off[fr]).
+// Block = 256 flat rows; threads stride over the block's contiguous output range (load balanced).
+template <typename T, typename RP>
+__global__ __launch_bounds__(kRowBlock) void k_compact(RowsDev<T, RP> R, const int64_t* dst_off, int* dst_col,
+                                                       T* dst_val, const int* src_col, const T* src_val) {
+  __shared__ long long s_dst[kRowBlock + 1];
+  __shared__ long long s_src[kRowBlock];
+  const int tid = threadIdx.x;
+  const int64_t fr0 = (int64_t)blockIdx.x * kRowBlock;
+  s_dst[tid] = dst_off[fr0 + tid];
+  if (tid == 0) s_dst[kRowBlock] = dst_off[fr0 + kRowBlock];
+  s_src[tid] = R.off[fr0 + tid];
+  __syncthreads();
+  const long long lo = s_dst[0], hi = s_dst[kRowBlock];
+  for (long long e = lo + tid; e < hi; e += kRowBlock) {
+    int a = 0, b = kRowBlock;  // find r with s_dst[r] <= e < s_dst[r+1]
+    while (b - a > 1) {
+      int mid = (a + b) >> 1;
+      if (s_dst[mid] <= e) a = mid; else b = mid;
+    }
+    const long long src = s_src[a] + (e - s_dst[a]);
+    dst_col[e] = src_col[src];
+    dst_val[e] = src_val[src];
+  }
+}
+
+// Per-product row_ptr from flat offsets: rp[i] = off[flat0 + i] - off[flat0] (int32 or int64 out).
+template <typename OUT>
+__global__ void k_rowptr(const int64_t* off, int64_t flat0, int64_t m, OUT* rp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= m) rp[i] = (OUT)(off[flat0 + i] - off[flat0]);
+}
+
+}  // namespace bclx

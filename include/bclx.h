@@ -1,0 +1,156 @@
+/* bclx -- MI355X-native local tile multiply for BCL's distributed gemm (C ABI).
+ *
+ * Every entry point takes plain pointers and sizes, returns a bclx_status and never throws. Pointers named
+ * d_* are device (HBM) pointers, h_* are host pointers. Calls are ordered on the handle's HIP stream;
+ * functions that report a size to the host (nnz of a product) synchronise that stream. One handle per host
+ * thread.
+ *
+ * Reference interfaces replaced (paths relative to the BCL source tree):
+ *   bclx_spgemm_symbolic + bclx_spgemm_numeric
+ *       BCL::gemm(const CSRMatrix&, const CSRMatrix&, CSRMatrix&)  bcl/containers/algorithms/spgemm.hpp:40-76
+ *       as reached through CSRMatrix::dot                          bcl/containers/sequential/CSRMatrix.hpp:229-235
+ *       (GPU counterpart: spgemm_cusparse                          bcl/containers/experimental/cuda/sequential/
+ *                                                                  cusparse_util.cuh:269-514)
+ *   bclx_csr_add
+ *       the C += part of BCL::gemm (pre-insert of C, spgemm.hpp:47-54) and the pairwise sum_cusparse
+ *       (cusparse_util.cuh:86-249)
+ *   bclx_accumulate_tiles
+ *       SparseHashAccumulator::get_matrix                          bcl/containers/sequential/SparseHashAccumulator.hpp:101-182
+ *   bclx_stage_*  (batched SUMMA stage)
+ *       the per-stage local products of BCL::gemm(SPMatrix, SPMatrix, SPMatrix)  spgemm.hpp:181-242
+ *   bclx_spmm
+ *       spmm_wrapper(m, n, k, nnz, c, b, a_values, a_rowptr, a_colind, ldb, ldc)  spgemm.hpp:140-149
+ *       (GPU counterpart: spmm_cusparse, cusparse_util.cuh:675-775)
+ *   bclx_gemm
+ *       cblas_gemm_wrapper_(layout, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+ *                                                                  bcl/containers/algorithms/cblas_wrapper.hpp:16-40
+ */
+#ifndef BCLX_H
+#define BCLX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BCLX_API __attribute__((visibility("default")))
+
+typedef enum {
+  BCLX_OK = 0,
+  BCLX_ERR_INVALID = 1,     /* bad argument (shape mismatch, null pointer, unsupported width) */
+  BCLX_ERR_HIP = 2,         /* a HIP runtime call failed */
+  BCLX_ERR_ALLOC = 3,       /* device allocation failed */
+  BCLX_ERR_OVERFLOW = 4,    /* a result does not fit the requested index width or a fixed capacity */
+  BCLX_ERR_UNSUPPORTED = 5  /* valid request this build does not implement */
+} bclx_status;
+
+typedef enum { BCLX_F32 = 0, BCLX_F64 = 1, BCLX_BF16 = 2 } bclx_dtype;
+
+/* 0-based CSR matrix in device memory. row_ptr has m + 1 entries of ptr_width bytes (4 or 8) starting at 0;
+ * col_idx holds int32 columns, sorted and unique within each row; vals holds nnz values of dtype. */
+typedef struct {
+  int64_t m, n, nnz;
+  int32_t ptr_width;
+  int32_t dtype;
+  void* row_ptr;
+  int32_t* col_idx;
+  void* vals;
+} bclx_csr;
+
+typedef struct bclx_handle bclx_handle;
+typedef struct bclx_spgemm_plan bclx_spgemm_plan;
+typedef struct bclx_stage_plan bclx_stage_plan;
+
+/* Last error message of the calling thread (empty string when none). */
+BCLX_API const char* bclx_last_error(void);
+
+/* Handle on `device` ordered on `stream` (a hipStream_t; NULL = the default stream). */
+BCLX_API bclx_status bclx_create(int device, void* stream, bclx_handle** out);
+BCLX_API bclx_status bclx_destroy(bclx_handle* h);
+BCLX_API bclx_status bclx_set_stream(bclx_handle* h, void* stream);
+BCLX_API bclx_status bclx_synchronize(bclx_handle* h);
+
+/* ---- SpGEMM C = A * B, two phases (exact-zero sums are kept, as BCL does) ---------------------------------
+ * symbolic: row analysis (products per row), binning and an exact per-row count of C's pattern; returns
+ * nnz(C). numeric: fills caller-allocated C (row_ptr m+1 of C->ptr_width, col_idx/vals of nnz) with sorted
+ * columns and values accumulated in float64 and rounded once to C->dtype. A and B must share dtype and
+ * ptr_width. C->ptr_width 4 fails with BCLX_ERR_OVERFLOW when nnz(C) > 2^31 - 1. */
+BCLX_API bclx_status bclx_spgemm_symbolic(bclx_handle* h, const bclx_csr* A, const bclx_csr* B,
+                                          bclx_spgemm_plan** plan, int64_t* nnz_c);
+BCLX_API bclx_status bclx_spgemm_numeric(bclx_spgemm_plan* plan, bclx_csr* C);
+BCLX_API bclx_status bclx_spgemm_plan_destroy(bclx_spgemm_plan* plan);
+/* Per-row products (int64, m entries) and, after symbolic, nnz(C_i) (int32) of a plan, copied to host. */
+BCLX_API bclx_status bclx_spgemm_plan_rows(bclx_spgemm_plan* plan, int64_t* h_products, int32_t* h_row_nnz);
+
+/* ---- sparse add / cross-stage accumulation --------------------------------------------------------------
+ * C = sum of ntiles m x n CSR matrices, summed per entry in tile order in float64; columns sorted.
+ * Two calls: with C->col_idx == NULL only C->nnz (and C->row_ptr when non-NULL) are produced. */
+BCLX_API bclx_status bclx_accumulate_tiles(bclx_handle* h, int ntiles, const bclx_csr* tiles, bclx_csr* C);
+
+/* ---- batched SUMMA stage ----------------------------------------------------------------------------------
+ * npairs products C_p = A_p * B_p run as one stage. mode 0 = plain (per-product two-phase launches, host
+ * synchronised like a per-tile library call); mode 1 = batched + learned: one persistent pipeline over all
+ * products, per-row output sizes predicted by the row-work MLP in place of the symbolic pass, overflow rows
+ * redone exactly, captured in a HIP graph and replayed by bclx_stage_execute. Outputs live in the plan. */
+typedef struct {
+  int32_t mode;             /* 0 plain, 1 batched + learned, 2 batched + exact symbolic (no predictor) */
+  int32_t use_graph;        /* mode 1/2: capture the pipeline in a HIP graph (1) or launch eagerly (0) */
+  float margin;             /* predicted capacity = ceil(pred * (1 + margin)) + slack */
+  int32_t slack;
+  const float* h_mlp_params; /* 5313 fp32 MLP parameters (NULL: mode 1 falls back to upper bounds) */
+  int32_t persistent_blocks; /* 0 = automatic */
+  const int32_t* h_tuning;   /* optional autotuned table (see bclx_tuning_size); NULL = defaults */
+} bclx_stage_opts;
+
+BCLX_API bclx_status bclx_stage_plan_create(bclx_handle* h, int npairs, const bclx_csr* A, const bclx_csr* B,
+                                            const bclx_stage_opts* opts, bclx_stage_plan** plan);
+BCLX_API bclx_status bclx_stage_execute(bclx_stage_plan* plan);
+/* Device view of product p's result (row_ptr int64, pointers into the plan's arena; valid until destroy). */
+BCLX_API bclx_status bclx_stage_result(bclx_stage_plan* plan, int p, bclx_csr* C);
+/* Stats: [0] total products, [1] total nnz(C), [2] overflow rows, [3] predicted rows, [4] arena capacity,
+ * [5] rows, [6] heavy rows. */
+BCLX_API bclx_status bclx_stage_stats(bclx_stage_plan* plan, int64_t* h_stats, int nstats);
+BCLX_API bclx_status bclx_stage_plan_destroy(bclx_stage_plan* plan);
+
+/* ---- learned row-work predictor (MLP 16 -> 64 -> 64 -> 1, ReLU) -------------------------------------------
+ * Parameter layout: W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3[1] = 5313 floats. */
+#define BCLX_MLP_NPARAM 5313
+#define BCLX_MLP_NFEAT 16
+/* Features of every row of C = A * B (d_feat: m x 16 fp32), plus products per row (d_prod int64, m). */
+BCLX_API bclx_status bclx_row_features(bclx_handle* h, const bclx_csr* A, const bclx_csr* B, float* d_feat,
+                                       int64_t* d_prod);
+BCLX_API bclx_status bclx_mlp_forward(bclx_handle* h, int64_t nrows, const float* d_x, const float* d_params,
+                                      float* d_y);
+/* One minibatch: d_grads (5313 fp32) = gradient of 0.5 * mean((y - t)^2); *d_loss = that loss (fp32). */
+BCLX_API bclx_status bclx_mlp_grad(bclx_handle* h, int64_t nrows, const float* d_x, const float* d_t,
+                                   const float* d_params, float* d_grads, float* d_loss);
+/* Adam step on d_params with moments d_m, d_v (5313 each), step number t >= 1. */
+BCLX_API bclx_status bclx_mlp_adam(bclx_handle* h, float* d_params, const float* d_grads, float* d_m, float* d_v,
+                                   int t, float lr, float beta1, float beta2, float eps);
+
+/* ---- SpMM: C[i*ldc + j] = beta * C + alpha * sum_k A[i,k] * B[k*ldb + j] for j < n -------------------------
+ * A fp32 CSR (ptr_width 4 or 8), B row-major fp32 (b_dtype BCLX_F32) or bf16 (BCLX_BF16), C row-major fp32,
+ * products accumulated in fp32. BCL's spmm_wrapper is alpha = beta = 1. */
+BCLX_API bclx_status bclx_spmm(bclx_handle* h, const bclx_csr* A, const void* d_B, int32_t b_dtype, int64_t ldb,
+                               float* d_C, int64_t ldc, int64_t n, float alpha, float beta);
+
+/* ---- dense GEMM: C = beta * C + alpha * op(A) * op(B), row-major (BCL calls it with alpha = beta = 1) --------
+ * mode 0: fp32 inputs, FP32 MFMA (xf32-free, exact fp32 products); mode 1: inputs rounded to bf16, BF16 MFMA
+ * with FP32 accumulation; mode 2: split-bf16 3-term (a_hi b_hi + a_hi b_lo + a_lo b_hi) on BF16 MFMA with FP32
+ * accumulation. A, B, C fp32. trans_a / trans_b: 0 = no transpose, 1 = transpose. */
+BCLX_API bclx_status bclx_gemm(bclx_handle* h, int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
+                               float alpha, const float* d_A, int64_t lda, const float* d_B, int64_t ldb, float beta,
+                               float* d_C, int64_t ldc, int32_t mode);
+/* FP64 dense GEMM C = A * B + C (fp32 inputs widened exactly, fp64 C): the GPU error oracle for large sizes. */
+BCLX_API bclx_status bclx_gemm_f64_ref(bclx_handle* h, int64_t m, int64_t n, int64_t k, const float* d_A,
+                                       int64_t lda, const float* d_B, int64_t ldb, double* d_C, int64_t ldc);
+
+/* Library version string. */
+BCLX_API const char* bclx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BCLX_H */
