@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-fvisibility=hidden", "-I" + os.path.join(ROOT, "include"), "-Wno-unused-result",
 ]
-HIP_SOURCES = ["core.hip", "spgemm.hip"]
+HIP_SOURCES = ["core.hip", "spgemm.hip", "stage.hip", "mlp.hip"]
 CPP_SOURCES = ["stubs.cpp"]
 
 

@@ -279,6 +279,8 @@ TileDev<T, RP> make_tile(const bclx_csr* A, const bclx_csr* B) {
   t.b_val = static_cast<const T*>(B->vals);
   t.m = (int32_t)A->m;
   t.n = (int32_t)B->n;
+  t.avg_a = A->m ? (float)((double)A->nnz / (double)A->m) : 0.f;
+  t.avg_b = B->m ? (float)((double)B->nnz / (double)B->m) : 0.f;
   return t;
 }
 template TileDev<float, int32_t> make_tile<float, int32_t>(const bclx_csr*, const bclx_csr*);
@@ -332,6 +334,7 @@ struct TwoPhase : PlanBase {
     hipStream_t s = h->stream;
     F.tiles[0].c_col = C->col_idx;
     F.tiles[0].c_val = static_cast<T*>(C->vals);
+    F.R.out_limit = C->nnz;
     F.upload_tiles();
     F.bin(s, 0);
     int32_t counts[kNumBins];

@@ -54,6 +54,7 @@ struct TileDev {
   T* c_val;
   int64_t flat0;  // first flat row of this product
   int32_t m, n;
+  float avg_a, avg_b;  // average row lengths of A and B (predictor features)
 };
 
 template <typename T, typename RP>
@@ -70,7 +71,8 @@ struct RowsDev {
   int32_t* ovf_count;        // overflow list (learned path)
   int32_t* ovf_items;
   unsigned long long* cursor;  // bump allocator for rows placed outside their predicted slot
-  unsigned long long* stats;   // [0] products [1] heavy rows [2] overflow rows
+  unsigned long long* stats;   // [0] products [1] heavy rows [2] overflow rows [3] capacity exceeded [4] predicted
+  long long out_limit;         // entries available in the output arrays; writes beyond it are refused
   BinTuning tn;
 };
 
@@ -352,8 +354,10 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
       }
       bitonic_sort_groups<TS, G, BLOCK>(keys, vals, lane, true);
       __syncthreads();
-      if (active && !ovf) {
-        const long long o = MODE == 3 ? s_base[g] : R.off[fr];
+      const long long o = MODE == 3 ? s_base[g] : (active ? R.off[fr] : 0);
+      const bool fits = o + cnt <= R.out_limit;
+      if (active && !ovf && !fits && lane == 0) R.stats[3] = 1ull;
+      if (active && !ovf && fits) {
         for (int s = lane; s < cnt; s += G) {
           tile.c_col[o + s] = keys[s];
           tile.c_val[o + s] = (T)vals[s];
@@ -434,20 +438,30 @@ __global__ __launch_bounds__(kHeavyThreads) void k_heavy(RowsDev<T, RP> R, int b
           b = (long long)atomicAdd(R.cursor, (unsigned long long)total);
           R.off[fr] = b;
         }
+        if (b + total > R.out_limit) {
+          R.stats[3] = 1ull;
+          b = -1;
+        }
         s_base = b;
         R.nnz[fr] = (int)total;
       }
       __syncthreads();
+      const bool fits = s_base >= 0;
       long long pos = s_base + pre;
       for (int64_t w = w0; w < w1; w++) {
         unsigned word = bits[w];
-        while (word) {
+        while (word && fits) {
           const int bit = __ffs(word) - 1;
           const int j = (int)(w * 32 + bit);
           tile.c_col[pos] = j;
           tile.c_val[pos] = (T)dense[j];
-          dense[j] = 0.0;
           pos++;
+          word &= word - 1;
+        }
+        word = bits[w];
+        while (word) {  // re-zero the accumulator whether or not the row was written
+          const int bit = __ffs(word) - 1;
+          dense[w * 32 + bit] = 0.0;
           word &= word - 1;
         }
         if (!LDS_BITMAP) bits[w] = 0u;
@@ -468,7 +482,7 @@ constexpr int kScanBlock = 1024;
 constexpr int kScanPer = 4;
 constexpr int kScanTile = kScanBlock * kScanPer;
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int32_t* cnt, int64_t n, long long* partial) {
+static __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int32_t* cnt, int64_t n, long long* partial) {
   __shared__ long long sm[kScanBlock / 64 + 1];
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
   long long s = 0;
@@ -480,7 +494,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int32_t* cnt, 
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_top(long long* partial, int64_t nblocks, int64_t* total_out) {
+static __global__ __launch_bounds__(kScanBlock) void k_scan_top(long long* partial, int64_t nblocks, int64_t* total_out) {
   __shared__ long long sm[kScanBlock / 64 + 1];
   long long run = 0;
   for (int64_t base = 0; base < nblocks; base += kScanBlock) {
@@ -494,7 +508,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_top(long long* partial, int
   if (threadIdx.x == 0 && total_out) *total_out = run;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_final(const int32_t* cnt, int64_t n, const long long* partial,
+static __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int32_t* cnt, int64_t n, const long long* partial,
                                                            int64_t* off, int64_t off_add) {
   __shared__ long long sm[kScanBlock / 64 + 1];
   const int64_t base = (int64_t)blockIdx.x * kScanTile;

@@ -1,0 +1,513 @@
+// Batched SUMMA stage: the npairs local products C_p = A_p * B_p of one stage of BCL::gemm(SPMatrix, SPMatrix,
+// SPMatrix) (bcl/containers/algorithms/spgemm.hpp:181-242, the `my_a.dot(my_b)` of every owner for one k).
+//
+//   mode 0  plain: every product runs the two-phase path with its own launches and host synchronisation, the way
+//           a per-tile library call (spgemm_cusparse, cusparse_util.cuh:269-514) is used by the reference driver.
+//   mode 2  batched exact: one pipeline over the flat rows of all products: analysis, symbolic, scan, numeric.
+//   mode 1  batched learned: analysis, MLP-predicted row capacities in place of the symbolic pass, numeric into
+//           predicted slots, exact one-pass redo of the rows that overflowed their slot, compaction to CSR.
+// Modes 1 and 2 are sized once at plan time by an eager run and then captured in a HIP graph (LDS bins run as
+// parallel graph branches), so bclx_stage_execute is one graph launch.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mlp.hpp"
+#include "spgemm.hpp"
+
+namespace bclx {
+
+constexpr int kPredictMinProducts = 32;  // rows with at most this many products use the exact bound min(prod, n)
+
+template <typename T, typename RP>
+__global__ __launch_bounds__(kRowBlock) void k_predict(RowsDev<T, RP> R, const float* params, float margin, int slack,
+                                                       int use_mlp) {
+  __shared__ float P[kNParam];
+  if (use_mlp)
+    for (int q = threadIdx.x; q < kNParam; q += blockDim.x) P[q] = params[q];
+  __syncthreads();
+  const int64_t fr = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
+  const TileDev<T, RP> tile = R.tiles[R.blk_tile[blockIdx.x]];
+  const int64_t i = fr - tile.flat0;
+  const bool valid = i >= 0 && i < tile.m;
+  const long long prod = R.prod[fr];
+  const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
+  long long cap = valid ? ub : 0;
+  int predicted = 0;
+  if (valid && use_mlp && prod > kPredictMinProducts) {
+    float f[kFeat];
+    row_features<RP>(tile.a_rp, tile.a_col, tile.b_rp, tile.b_col, i, tile.n, 0, tile.avg_a, tile.avg_b, prod, f);
+    const float y = mlp_forward_row(P, f);
+    const double pred = exp2((double)fminf(fmaxf(y, 0.f), 40.f)) - 1.0;
+    const long long c = (long long)ceil(pred * (1.0 + (double)margin)) + slack;
+    cap = c < ub ? c : ub;
+    if (cap < 1) cap = 1;
+    predicted = 1;
+  }
+  R.cap[fr] = (int32_t)cap;
+  if (predicted) atomicAdd(&R.stats[4], 1ull);
+}
+
+// Overflow rows of the predicted pass -> LDS bins sized by the exact upper bound min(prod, n).
+template <typename T, typename RP>
+__global__ __launch_bounds__(256) void k_ovf_bin(RowsDev<T, RP> R) {
+  const int cnt = *R.ovf_count;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cnt; idx += gridDim.x * blockDim.x) {
+    const int fr = R.ovf_items[idx];
+    const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
+    const long long prod = R.prod[fr];
+    const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
+    const int b = bin_for(ub, prod, R.tn);
+    const int pos = atomicAdd(&R.bin_count[b], 1);
+    R.bin_items[(int64_t)b * R.bin_stride + pos] = fr;
+  }
+}
+
+// Per-product row_ptr (int64, starting at 0) from flat offsets: product t's row_ptr lives at rp[flat0 + t ...].
+template <typename T, typename RP>
+__global__ __launch_bounds__(kRowBlock) void k_tile_rowptr(RowsDev<T, RP> R, const int64_t* off, int64_t* rp) {
+  const int64_t fr = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
+  const int t = R.blk_tile[blockIdx.x];
+  const TileDev<T, RP> tile = R.tiles[t];
+  const int64_t i = fr - tile.flat0;
+  const int64_t base = off[tile.flat0];
+  if (i < tile.m) rp[fr + t] = off[fr] - base;
+  if (i == tile.m) rp[fr + t] = off[fr] - base;  // i == m is inside the padding, or the first row of the next block
+  if (tile.m % kRowBlock == 0 && i == tile.m - 1) rp[fr + t + 1] = off[fr + 1] - base;
+  if (tile.m == 0 && i == 0) rp[fr + t] = 0;
+}
+
+struct StageBase {
+  virtual ~StageBase() = default;
+  virtual void execute() = 0;
+  virtual void result(int p, bclx_csr* C) = 0;
+  virtual void stats(int64_t* out, int n) = 0;
+};
+
+template <typename T, typename RP>
+struct StagePlan : StageBase {
+  bclx_handle* h;
+  int mode, npairs;
+  bclx_stage_opts opts{};
+  BinTuning tn;
+  // mode 0
+  std::vector<std::unique_ptr<FlatRows<T, RP>>> per;  // one workspace per product
+  std::vector<DevBuf> c_rp, c_col, c_val;
+  std::vector<int64_t> c_nnz;
+  // modes 1 / 2
+  FlatRows<T, RP> F;
+  DevBuf params, arena_col, arena_val, final_col, final_val, final_off, tile_rp;
+  int64_t arena_cap = 0, final_cap = 0, main_cap = 0;
+  int32_t counts1[kNumBins] = {0}, counts2[kNumBins] = {0};
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> ev;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap_stream = nullptr;
+  std::vector<int64_t> host_stats;
+
+  StagePlan(bclx_handle* h_, int np, const bclx_csr* A, const bclx_csr* B, const bclx_stage_opts* o)
+      : h(h_), npairs(np) {
+    if (o) opts = *o;
+    mode = opts.mode;
+    if (opts.h_tuning) {
+      tn.work_per_lane = std::max(1, opts.h_tuning[0]);
+      tn.heavy_need = std::min(std::max(64, opts.h_tuning[1]), 4096);
+      tn.load_shift = std::min(std::max(1, opts.h_tuning[2]), 2);
+    }
+    std::vector<TileDev<T, RP>> tiles;
+    for (int p = 0; p < np; p++) tiles.push_back(make_tile<T, RP>(&A[p], &B[p]));
+    if (mode == 0) {
+      setup_plain(tiles);
+    } else {
+      for (int q = 0; q < 4; q++) {
+        hipStream_t s;
+        BCLX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        side.push_back(s);
+      }
+      for (int q = 0; q < 2 * (int)side.size() + 2; q++) {
+        hipEvent_t e;
+        BCLX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ev.push_back(e);
+      }
+      setup_batched(tiles);
+    }
+  }
+
+  ~StagePlan() override {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto s : side) (void)hipStreamDestroy(s);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+
+  // ---- mode 0 ----------------------------------------------------------------------------------------------
+  void plain_product(int p, bool allocate) {
+    FlatRows<T, RP>& W = *per[p];
+    hipStream_t s = h->stream;
+    W.reset_counters(s);
+    W.analyze(s, true);
+    int32_t counts[kNumBins];
+    BCLX_HIP(hipMemcpyAsync(counts, W.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
+    BCLX_HIP(hipStreamSynchronize(s));
+    for (int b = 1; b < kNumBins; b++) W.run_bin(s, 0, b, counts[b]);
+    W.scan(s, W.R.nnz, W.R.off, 0);
+    int64_t tot = 0;
+    BCLX_HIP(hipMemcpyAsync(&tot, W.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    BCLX_HIP(hipStreamSynchronize(s));
+    if (allocate) {
+      c_nnz[p] = tot;
+      c_col[p].alloc(sizeof(int) * std::max<int64_t>(tot, 1));
+      c_val[p].alloc(sizeof(T) * std::max<int64_t>(tot, 1));
+      c_rp[p].alloc(sizeof(int64_t) * (W.tiles[0].m + 1));
+      W.tiles[0].c_col = c_col[p].as<int>();
+      W.tiles[0].c_val = c_val[p].as<T>();
+      W.upload_tiles();
+    }
+    BCLX_REQUIRE(tot <= c_nnz[p], BCLX_ERR_OVERFLOW, "stage product grew since the plan was made");
+    W.R.out_limit = c_nnz[p];
+    W.bin(s, 0);
+    BCLX_HIP(hipMemcpyAsync(counts, W.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
+    BCLX_HIP(hipStreamSynchronize(s));
+    for (int b = 1; b < kNumBins; b++) W.run_bin(s, 1, b, counts[b]);
+    hipLaunchKernelGGL(k_rowptr<int64_t>, dim3(div_up(W.tiles[0].m + 1, 256)), dim3(256), 0, s, W.R.off, (int64_t)0,
+                       (int64_t)W.tiles[0].m, c_rp[p].as<int64_t>());
+    BCLX_HIP(hipGetLastError());
+  }
+
+  void setup_plain(const std::vector<TileDev<T, RP>>& tiles) {
+    c_rp.resize(npairs);
+    c_col.resize(npairs);
+    c_val.resize(npairs);
+    c_nnz.assign(npairs, 0);
+    for (int p = 0; p < npairs; p++) {
+      per.push_back(std::make_unique<FlatRows<T, RP>>());
+      per.back()->setup(h, {tiles[p]}, false);
+      per.back()->R.tn = tn;
+    }
+    for (int p = 0; p < npairs; p++) plain_product(p, true);  // sizes outputs
+    BCLX_HIP(hipStreamSynchronize(h->stream));
+  }
+
+  // ---- modes 1 / 2 ------------------------------------------------------------------------------------------
+  void fork(hipStream_t s) {
+    BCLX_HIP(hipEventRecord(ev[0], s));
+    for (auto q : side) BCLX_HIP(hipStreamWaitEvent(q, ev[0], 0));
+  }
+  void join(hipStream_t s) {
+    for (size_t q = 0; q < side.size(); q++) {
+      BCLX_HIP(hipEventRecord(ev[1 + q], side[q]));
+      BCLX_HIP(hipStreamWaitEvent(s, ev[1 + q], 0));
+    }
+  }
+  // Run the LDS bins and the heavy bin of `mode_k` on the side streams (largest expected first).
+  void run_bins(hipStream_t s, int mode_k, const int32_t* counts, bool persistent) {
+    fork(s);
+    int q = 0;
+    if (counts[kHeavyBin] > 0) F.run_bin(side[q++ % side.size()], mode_k, kHeavyBin, persistent ? -1 : counts[kHeavyBin]);
+    for (int b = 9; b >= 1; b--) {
+      if (counts[b] == 0) continue;
+      F.run_bin(side[q++ % side.size()], mode_k, b, counts[b]);
+    }
+    join(s);
+  }
+
+  void enqueue(hipStream_t s, bool persistent) {
+    F.reset_counters(s);
+    if (mode == 1) {
+      fill_i32(s, F.R.nnz, F.nflat, 0);
+      fill_u64(s, F.R.cursor, 1, (unsigned long long)main_cap);
+      F.analyze(s, false);
+      hipLaunchKernelGGL((k_predict<T, RP>), dim3((unsigned)F.nblocks), dim3(kRowBlock), 0, s, F.R,
+                         params.as<float>(), opts.margin, opts.slack, opts.h_mlp_params ? 1 : 0);
+      BCLX_HIP(hipGetLastError());
+      F.scan(s, F.R.cap, F.R.off, 0);
+      F.bin(s, 1);
+      run_bins(s, 2, counts1, persistent);
+      fill_i32(s, F.R.bin_count, kNumBins, 0);
+      hipLaunchKernelGGL((k_ovf_bin<T, RP>), dim3(std::max<unsigned>(1, std::min<unsigned>(div_up(counts_ovf, 256), 1024))),
+                         dim3(256), 0, s, F.R);
+      BCLX_HIP(hipGetLastError());
+      run_bins(s, 3, counts2, persistent);
+      F.scan(s, F.R.nnz, final_off.as<int64_t>(), 0);
+      F.compact(s, final_off.as<int64_t>(), final_col.as<int>(), final_val.as<T>(), arena_col.as<int>(),
+                arena_val.as<T>());
+    } else {
+      F.analyze(s, true);
+      run_bins(s, 0, counts1, persistent);
+      F.scan(s, F.R.nnz, F.R.off, 0);
+      F.bin(s, 0);
+      run_bins(s, 1, counts2, persistent);
+    }
+    const int64_t* off = mode == 1 ? final_off.as<int64_t>() : F.R.off;
+    hipLaunchKernelGGL((k_tile_rowptr<T, RP>), dim3((unsigned)F.nblocks), dim3(kRowBlock), 0, s, F.R, off,
+                       tile_rp.as<int64_t>());
+    BCLX_HIP(hipGetLastError());
+  }
+
+  unsigned counts_ovf = 0;
+
+  void read_counts(int32_t* dst) {
+    BCLX_HIP(hipMemcpyAsync(dst, F.R.bin_count, sizeof(int32_t) * kNumBins, hipMemcpyDeviceToHost, h->stream));
+    BCLX_HIP(hipStreamSynchronize(h->stream));
+  }
+
+  int64_t read_stat(int q) {
+    unsigned long long v = 0;
+    BCLX_HIP(hipMemcpyAsync(&v, F.R.stats + q, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    BCLX_HIP(hipStreamSynchronize(h->stream));
+    return (int64_t)v;
+  }
+
+  // Eager sizing run: the same steps as enqueue, with host reads of the bin counts and totals in between.
+  void sizing_run() {
+    hipStream_t s = h->stream;
+    F.reset_counters(s);
+    if (mode == 1) {
+      fill_i32(s, F.R.nnz, F.nflat, 0);
+      F.analyze(s, false);
+      hipLaunchKernelGGL((k_predict<T, RP>), dim3((unsigned)F.nblocks), dim3(kRowBlock), 0, s, F.R,
+                         params.as<float>(), opts.margin, opts.slack, opts.h_mlp_params ? 1 : 0);
+      BCLX_HIP(hipGetLastError());
+      F.scan(s, F.R.cap, F.R.off, 0);
+      int64_t tot_cap = 0;
+      BCLX_HIP(hipMemcpyAsync(&tot_cap, F.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      BCLX_HIP(hipStreamSynchronize(s));
+      main_cap = tot_cap;
+      // Room for rows that overflow their predicted slot; grown and re-run when this input needs more.
+      int64_t reserve = std::max<int64_t>(tot_cap / 8, 1 << 20);
+      bool ok = false;
+      for (int attempt = 0; attempt < 4 && !ok; attempt++) {
+        arena_cap = main_cap + reserve;
+        arena_col.alloc(sizeof(int) * arena_cap);
+        arena_val.alloc(sizeof(T) * arena_cap);
+        for (auto& t : F.tiles) {
+          t.c_col = arena_col.as<int>();
+          t.c_val = arena_val.as<T>();
+        }
+        F.upload_tiles();
+        F.R.out_limit = arena_cap;
+        fill_u64(s, F.R.cursor, 1, (unsigned long long)main_cap);
+        fill_u64(s, F.R.stats + 2, 2, 0);
+        fill_i32(s, F.R.ovf_count, 2, 0);
+        fill_i32(s, F.R.nnz, F.nflat, 0);
+        F.bin(s, 1);
+        read_counts(counts1);
+        for (int b = 1; b < kNumBins; b++) F.run_bin(s, 2, b, counts1[b]);
+        int32_t novf = 0;
+        BCLX_HIP(hipMemcpyAsync(&novf, F.R.ovf_count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        BCLX_HIP(hipStreamSynchronize(s));
+        counts_ovf = (unsigned)novf;
+        fill_i32(s, F.R.bin_count, kNumBins, 0);
+        hipLaunchKernelGGL((k_ovf_bin<T, RP>), dim3(std::max<unsigned>(1, std::min<unsigned>(div_up(novf, 256), 1024))),
+                           dim3(256), 0, s, F.R);
+        BCLX_HIP(hipGetLastError());
+        read_counts(counts2);
+        for (int b = 1; b < kNumBins; b++) F.run_bin(s, 3, b, counts2[b]);
+        unsigned long long cur = 0;
+        BCLX_HIP(hipMemcpyAsync(&cur, F.R.cursor, sizeof(cur), hipMemcpyDeviceToHost, s));
+        BCLX_HIP(hipStreamSynchronize(s));
+        // the cursor counts every request, granted or refused, so cur - main_cap is what this input needs
+        ok = read_stat(3) == 0;
+        if (!ok) reserve = ((int64_t)cur - main_cap) + ((int64_t)cur - main_cap) / 16 + 4096;
+      }
+      BCLX_REQUIRE(ok, BCLX_ERR_OVERFLOW, "could not size the overflow region of the stage");
+      F.scan(s, F.R.nnz, F.R.off + 0, 0);  // total nnz only (off is rebuilt by the pipeline)
+      int64_t tot = 0;
+      BCLX_HIP(hipMemcpyAsync(&tot, F.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      BCLX_HIP(hipStreamSynchronize(s));
+      final_cap = tot + tot / 64 + 4096;
+      final_col.alloc(sizeof(int) * final_cap);
+      final_val.alloc(sizeof(T) * final_cap);
+    } else {
+      F.analyze(s, true);
+      read_counts(counts1);
+      for (int b = 1; b < kNumBins; b++) F.run_bin(s, 0, b, counts1[b]);
+      F.scan(s, F.R.nnz, F.R.off, 0);
+      int64_t tot = 0;
+      BCLX_HIP(hipMemcpyAsync(&tot, F.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      BCLX_HIP(hipStreamSynchronize(s));
+      final_cap = tot + tot / 64 + 4096;
+      final_col.alloc(sizeof(int) * final_cap);
+      final_val.alloc(sizeof(T) * final_cap);
+      for (auto& t : F.tiles) {
+        t.c_col = final_col.as<int>();
+        t.c_val = final_val.as<T>();
+      }
+      F.upload_tiles();
+      F.R.out_limit = final_cap;
+      F.bin(s, 0);
+      read_counts(counts2);
+    }
+    BCLX_HIP(hipStreamSynchronize(s));
+  }
+
+  void setup_batched(const std::vector<TileDev<T, RP>>& tiles) {
+    F.setup(h, tiles, mode == 1);
+    F.R.tn = tn;
+    params.alloc(sizeof(float) * kNParam);
+    if (mode == 1 && opts.h_mlp_params)
+      BCLX_HIP(hipMemcpy(params.p, opts.h_mlp_params, sizeof(float) * kNParam, hipMemcpyHostToDevice));
+    final_off.alloc(sizeof(int64_t) * (F.nflat + 1));
+    tile_rp.alloc(sizeof(int64_t) * (F.nflat + npairs + 1));
+    // heavy-row accumulators are allocated before capture
+    ensure_dense_slots(h, h->num_cus, (size_t)F.max_n);
+    sizing_run();
+    if (opts.use_graph) {
+      // capture on a plan-owned stream (the handle's stream may be the legacy default stream, which cannot be
+      // captured); the graph is launched on the handle's stream
+      BCLX_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+      hipStream_t s = cap_stream;
+      BCLX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(s, false);
+      } catch (...) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(s, &g);
+        throw;
+      }
+      BCLX_HIP(hipStreamEndCapture(s, &graph));
+      BCLX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+  }
+
+  void execute() override {
+    if (mode == 0) {
+      for (int p = 0; p < npairs; p++) plain_product(p, false);
+      return;
+    }
+    if (exec) BCLX_HIP(hipGraphLaunch(exec, h->stream));
+    else enqueue(h->stream, false);
+  }
+
+  void result(int p, bclx_csr* C) override {
+    BCLX_REQUIRE(p >= 0 && p < npairs, BCLX_ERR_INVALID, "product index out of range");
+    BCLX_HIP(hipStreamSynchronize(h->stream));
+    C->ptr_width = 8;
+    C->dtype = std::is_same<T, float>::value ? BCLX_F32 : BCLX_F64;
+    if (mode == 0) {
+      C->m = per[p]->tiles[0].m;
+      C->n = per[p]->tiles[0].n;
+      C->nnz = c_nnz[p];
+      C->row_ptr = c_rp[p].p;
+      C->col_idx = c_col[p].as<int>();
+      C->vals = c_val[p].p;
+      return;
+    }
+    const TileDev<T, RP>& t = F.tiles[p];
+    const int64_t* off = mode == 1 ? final_off.as<int64_t>() : F.R.off;
+    int64_t lo = 0, hi = 0;
+    BCLX_HIP(hipMemcpy(&lo, off + t.flat0, sizeof(int64_t), hipMemcpyDeviceToHost));
+    BCLX_HIP(hipMemcpy(&hi, off + t.flat0 + t.m, sizeof(int64_t), hipMemcpyDeviceToHost));
+    C->m = t.m;
+    C->n = t.n;
+    C->nnz = hi - lo;
+    C->row_ptr = tile_rp.as<int64_t>() + t.flat0 + p;
+    C->col_idx = final_col.as<int>() + lo;
+    C->vals = final_val.as<T>() + lo;
+  }
+
+  void stats(int64_t* out, int n) override {
+    BCLX_HIP(hipStreamSynchronize(h->stream));
+    int64_t st[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t rows = 0;
+    if (mode == 0) {
+      for (int p = 0; p < npairs; p++) {
+        unsigned long long v[2];
+        BCLX_HIP(hipMemcpy(v, per[p]->R.stats, sizeof(v), hipMemcpyDeviceToHost));
+        st[0] += (int64_t)v[0];
+        st[6] += (int64_t)v[1];
+        st[1] += c_nnz[p];
+        rows += per[p]->tiles[0].m;
+      }
+      st[4] = 0;
+    } else {
+      unsigned long long v[5];
+      BCLX_HIP(hipMemcpy(v, F.R.stats, sizeof(v), hipMemcpyDeviceToHost));
+      const int64_t* off = mode == 1 ? final_off.as<int64_t>() : F.R.off;
+      int64_t tot = 0;
+      BCLX_HIP(hipMemcpy(&tot, off + F.nflat, sizeof(int64_t), hipMemcpyDeviceToHost));
+      st[0] = (int64_t)v[0];
+      st[1] = tot;
+      st[2] = (int64_t)v[2];
+      st[3] = (int64_t)v[4];
+      st[4] = mode == 1 ? arena_cap : final_cap;
+      st[6] = (int64_t)v[1];
+      BCLX_REQUIRE(v[3] == 0, BCLX_ERR_OVERFLOW,
+                   "stage output exceeded the capacity sized at plan time (inputs changed since the plan?)");
+      for (auto& t : F.tiles) rows += t.m;
+    }
+    st[5] = rows;
+    for (int q = 0; q < n && q < 7; q++) out[q] = st[q];
+  }
+};
+
+}  // namespace bclx
+
+struct bclx_stage_plan {
+  bclx_handle* h;
+  std::unique_ptr<bclx::StageBase> impl;
+};
+
+using namespace bclx;
+
+extern "C" {
+
+bclx_status bclx_stage_plan_create(bclx_handle* h, int npairs, const bclx_csr* A, const bclx_csr* B,
+                                   const bclx_stage_opts* opts, bclx_stage_plan** plan) {
+  return guarded([&] {
+    BCLX_REQUIRE(h && A && B && plan && npairs > 0, BCLX_ERR_INVALID, "null argument or npairs <= 0");
+    const int mode = opts ? opts->mode : 0;
+    BCLX_REQUIRE(mode >= 0 && mode <= 2, BCLX_ERR_INVALID, "mode must be 0, 1 or 2");
+    for (int p = 0; p < npairs; p++) {
+      check_pair(&A[p], &B[p]);
+      BCLX_REQUIRE(A[p].dtype == A[0].dtype && A[p].ptr_width == A[0].ptr_width, BCLX_ERR_INVALID,
+                   "all products of a stage must share dtype and ptr_width");
+    }
+    set_device(h);
+    auto p = std::make_unique<bclx_stage_plan>();
+    p->h = h;
+    const bool f64 = A[0].dtype == BCLX_F64, w8 = A[0].ptr_width == 8;
+    if (!f64 && !w8) p->impl = std::make_unique<StagePlan<float, int32_t>>(h, npairs, A, B, opts);
+    else if (!f64 && w8) p->impl = std::make_unique<StagePlan<float, int64_t>>(h, npairs, A, B, opts);
+    else if (f64 && !w8) p->impl = std::make_unique<StagePlan<double, int32_t>>(h, npairs, A, B, opts);
+    else p->impl = std::make_unique<StagePlan<double, int64_t>>(h, npairs, A, B, opts);
+    *plan = p.release();
+  });
+}
+
+bclx_status bclx_stage_execute(bclx_stage_plan* plan) {
+  return guarded([&] {
+    BCLX_REQUIRE(plan, BCLX_ERR_INVALID, "null plan");
+    set_device(plan->h);
+    plan->impl->execute();
+  });
+}
+
+bclx_status bclx_stage_result(bclx_stage_plan* plan, int p, bclx_csr* C) {
+  return guarded([&] {
+    BCLX_REQUIRE(plan && C, BCLX_ERR_INVALID, "null argument");
+    set_device(plan->h);
+    plan->impl->result(p, C);
+  });
+}
+
+bclx_status bclx_stage_stats(bclx_stage_plan* plan, int64_t* h_stats, int nstats) {
+  return guarded([&] {
+    BCLX_REQUIRE(plan && h_stats, BCLX_ERR_INVALID, "null argument");
+    set_device(plan->h);
+    plan->impl->stats(h_stats, nstats);
+  });
+}
+
+bclx_status bclx_stage_plan_destroy(bclx_stage_plan* plan) {
+  return guarded([&] {
+    if (!plan) return;
+    set_device(plan->h);
+    (void)hipStreamSynchronize(plan->h->stream);
+    delete plan;
+  });
+}
+
+}  // extern "C"

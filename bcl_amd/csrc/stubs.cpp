@@ -15,22 +15,6 @@ void set_error(const std::string& msg);
 extern "C" {
 
 bclx_status bclx_accumulate_tiles(bclx_handle*, int, const bclx_csr*, bclx_csr*) { BCLX_STUB("bclx_accumulate_tiles"); }
-bclx_status bclx_stage_plan_create(bclx_handle*, int, const bclx_csr*, const bclx_csr*, const bclx_stage_opts*,
-                                   bclx_stage_plan**) { BCLX_STUB("bclx_stage_plan_create"); }
-bclx_status bclx_stage_execute(bclx_stage_plan*) { BCLX_STUB("bclx_stage_execute"); }
-bclx_status bclx_stage_result(bclx_stage_plan*, int, bclx_csr*) { BCLX_STUB("bclx_stage_result"); }
-bclx_status bclx_stage_stats(bclx_stage_plan*, int64_t*, int) { BCLX_STUB("bclx_stage_stats"); }
-bclx_status bclx_stage_plan_destroy(bclx_stage_plan*) { BCLX_STUB("bclx_stage_plan_destroy"); }
-bclx_status bclx_row_features(bclx_handle*, const bclx_csr*, const bclx_csr*, float*, int64_t*) {
-  BCLX_STUB("bclx_row_features");
-}
-bclx_status bclx_mlp_forward(bclx_handle*, int64_t, const float*, const float*, float*) { BCLX_STUB("bclx_mlp_forward"); }
-bclx_status bclx_mlp_grad(bclx_handle*, int64_t, const float*, const float*, const float*, float*, float*) {
-  BCLX_STUB("bclx_mlp_grad");
-}
-bclx_status bclx_mlp_adam(bclx_handle*, float*, const float*, float*, float*, int, float, float, float, float) {
-  BCLX_STUB("bclx_mlp_adam");
-}
 bclx_status bclx_spmm(bclx_handle*, const bclx_csr*, const void*, int32_t, int64_t, float*, int64_t, int64_t, float,
                       float) { BCLX_STUB("bclx_spmm"); }
 bclx_status bclx_gemm(bclx_handle*, int32_t, int32_t, int64_t, int64_t, int64_t, float, const float*, int64_t,
