@@ -182,7 +182,22 @@ template <typename T, typename RP>
 void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint) {
   // count_hint >= 0: exact row count known on the host (grid sized to it); < 0: persistent grid.
   if (b == 0 || count_hint == 0) return;
-  if (b == kHeavyBin) {
+  if (b == kHeavyBin && (max_n + 31) / 32 <= kHRWords) {
+    const int nslots = (int)std::max<int64_t>(1, std::min<int64_t>(count_hint >= 0 ? count_hint : h->num_cus,
+                                                                     (int64_t)h->num_cus));
+    ensure_dense_slots(h, nslots, (size_t)max_n);
+    double* acc = h->dense_slots.as<double>();
+    const int64_t sd = (int64_t)h->dense_slot_doubles;
+    switch (mode) {
+      case 0: hipLaunchKernelGGL((k_heavy_rank<0, T, RP>), dim3(nslots), dim3(kHRBlock), 0, s, R, kHeavyBin, acc, sd); break;
+      case 1: hipLaunchKernelGGL((k_heavy_rank<1, T, RP>), dim3(nslots), dim3(kHRBlock), 0, s, R, kHeavyBin, acc, sd); break;
+      case 2: hipLaunchKernelGGL((k_heavy_rank<2, T, RP>), dim3(nslots), dim3(kHRBlock), 0, s, R, kHeavyBin, acc, sd); break;
+      default: hipLaunchKernelGGL((k_heavy_rank<3, T, RP>), dim3(nslots), dim3(kHRBlock), 0, s, R, kHeavyBin, acc, sd); break;
+    }
+    BCLX_HIP(hipGetLastError());
+    return;
+  }
+  if (b == kHeavyBin) {  // global dense accumulator (tiles too wide for a window index)
     const int64_t words = (max_n + 31) / 32;
     const bool lds_bitmap = words * 4 <= (int64_t)h->lds_per_block - 4096;
     int nslots = (int)std::min<int64_t>(count_hint >= 0 ? count_hint : h->num_cus, (int64_t)h->num_cus);
@@ -237,6 +252,20 @@ void FlatRows<T, RP>::compact(hipStream_t s, const int64_t* dst_off, int* dst_co
   BCLX_HIP(hipGetLastError());
 }
 
+template <typename T, typename RP>
+int64_t FlatRows<T, RP>::read_stat(int q) {
+  unsigned long long v = 0;
+  BCLX_HIP(hipMemcpyAsync(&v, R.stats + q, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+  BCLX_HIP(hipStreamSynchronize(h->stream));
+  return (int64_t)v;
+}
+
+template <typename T, typename RP>
+void FlatRows<T, RP>::prepare_heavy() {
+  // rank-compact kernel: one scratch of max_n doubles per workgroup; global dense fallback: the same shape
+  ensure_dense_slots(h, h->num_cus, (size_t)max_n);
+}
+
 template struct FlatRows<float, int32_t>;
 template struct FlatRows<float, int64_t>;
 template struct FlatRows<double, int32_t>;
@@ -279,6 +308,7 @@ TileDev<T, RP> make_tile(const bclx_csr* A, const bclx_csr* B) {
   t.b_val = static_cast<const T*>(B->vals);
   t.m = (int32_t)A->m;
   t.n = (int32_t)B->n;
+  t.krows = B->m;
   t.avg_a = A->m ? (float)((double)A->nnz / (double)A->m) : 0.f;
   t.avg_b = B->m ? (float)((double)B->nnz / (double)B->m) : 0.f;
   return t;
@@ -288,6 +318,57 @@ template TileDev<float, int64_t> make_tile<float, int64_t>(const bclx_csr*, cons
 template TileDev<double, int32_t> make_tile<double, int32_t>(const bclx_csr*, const bclx_csr*);
 template TileDev<double, int64_t> make_tile<double, int64_t>(const bclx_csr*, const bclx_csr*);
 
+template <typename T, typename RP>
+int64_t TwoPhase<T, RP>::symbolic() {
+  hipStream_t s = h->stream;
+  F.reset_counters(s);
+  F.analyze(s, true);
+  int32_t counts[kNumBins];
+  BCLX_HIP(hipMemcpyAsync(counts, F.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
+  BCLX_HIP(hipStreamSynchronize(s));
+  if (counts[kHeavyBin] > 0) F.prepare_heavy();
+  for (int b = 1; b < kNumBins; b++) F.run_bin(s, 0, b, counts[b]);
+  F.scan(s, F.R.nnz, F.R.off, 0);
+  int64_t tot = 0;
+  BCLX_HIP(hipMemcpyAsync(&tot, F.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  BCLX_HIP(hipStreamSynchronize(s));
+  nnz_c = tot;
+  return tot;
+}
+
+template <typename T, typename RP>
+void TwoPhase<T, RP>::numeric(int* c_col, T* c_val, int64_t c_cap, void* row_ptr, int ptr_width) {
+  BCLX_REQUIRE(nnz_c >= 0, BCLX_ERR_INVALID, "numeric phase before the symbolic phase");
+  BCLX_REQUIRE(c_cap >= nnz_c, BCLX_ERR_INVALID, "C arrays smaller than nnz from the symbolic phase");
+  BCLX_REQUIRE(ptr_width == 8 || nnz_c <= INT32_MAX, BCLX_ERR_OVERFLOW,
+               "nnz(C) = " + std::to_string(nnz_c) + " does not fit an int32 row_ptr; request ptr_width 8");
+  hipStream_t s = h->stream;
+  if (F.tiles[0].c_col != c_col || F.tiles[0].c_val != c_val) {
+    F.tiles[0].c_col = c_col;
+    F.tiles[0].c_val = c_val;
+    F.upload_tiles();
+  }
+  F.R.out_limit = c_cap;
+  F.bin(s, 0);
+  int32_t counts[kNumBins];
+  BCLX_HIP(hipMemcpyAsync(counts, F.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
+  BCLX_HIP(hipStreamSynchronize(s));
+  for (int b = 1; b < kNumBins; b++) F.run_bin(s, 1, b, counts[b]);
+  const unsigned g = div_up(m + 1, 256);
+  if (ptr_width == 8)
+    hipLaunchKernelGGL(k_rowptr<int64_t>, dim3(g), dim3(256), 0, s, F.R.off, (int64_t)0, m,
+                       static_cast<int64_t*>(row_ptr));
+  else
+    hipLaunchKernelGGL(k_rowptr<int32_t>, dim3(g), dim3(256), 0, s, F.R.off, (int64_t)0, m,
+                       static_cast<int32_t*>(row_ptr));
+  BCLX_HIP(hipGetLastError());
+}
+
+template struct TwoPhase<float, int32_t>;
+template struct TwoPhase<float, int64_t>;
+template struct TwoPhase<double, int32_t>;
+template struct TwoPhase<double, int64_t>;
+
 struct PlanBase {
   virtual ~PlanBase() = default;
   virtual int64_t symbolic() = 0;
@@ -296,64 +377,23 @@ struct PlanBase {
 };
 
 template <typename T, typename RP>
-struct TwoPhase : PlanBase {
-  bclx_handle* h;
-  FlatRows<T, RP> F;
-  int64_t nnz_c = -1;
-  int64_t m = 0;
-  TwoPhase(bclx_handle* h_, const bclx_csr* A, const bclx_csr* B) : h(h_) {
-    std::vector<TileDev<T, RP>> tiles{make_tile<T, RP>(A, B)};
-    F.setup(h, tiles, false);
-    m = A->m;
-  }
-  int64_t symbolic() override {
-    hipStream_t s = h->stream;
-    F.reset_counters(s);
-    F.analyze(s, true);
-    int32_t counts[kNumBins];
-    BCLX_HIP(hipMemcpyAsync(counts, F.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
-    BCLX_HIP(hipStreamSynchronize(s));
-    for (int b = 1; b < kNumBins; b++) F.run_bin(s, 0, b, counts[b]);
-    F.scan(s, F.R.nnz, F.R.off, 0);
-    int64_t tot = 0;
-    BCLX_HIP(hipMemcpyAsync(&tot, F.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    BCLX_HIP(hipStreamSynchronize(s));
-    nnz_c = tot;
-    return tot;
-  }
+struct TwoPhasePlan : PlanBase {
+  TwoPhase<T, RP> tp;
+  TwoPhasePlan(bclx_handle* h, const bclx_csr* A, const bclx_csr* B) : tp(h, make_tile<T, RP>(A, B), BinTuning{}) {}
+  int64_t symbolic() override { return tp.symbolic(); }
   void numeric(bclx_csr* C) override {
-    BCLX_REQUIRE(nnz_c >= 0, BCLX_ERR_INVALID, "bclx_spgemm_numeric before bclx_spgemm_symbolic");
     BCLX_REQUIRE(C && C->row_ptr, BCLX_ERR_INVALID, "C or C->row_ptr is null");
-    BCLX_REQUIRE(C->m == m, BCLX_ERR_INVALID, "C has the wrong number of rows");
-    BCLX_REQUIRE(C->nnz >= nnz_c, BCLX_ERR_INVALID, "C arrays smaller than nnz from the symbolic phase");
-    BCLX_REQUIRE(nnz_c == 0 || (C->col_idx && C->vals), BCLX_ERR_INVALID, "C->col_idx / C->vals are null");
-    BCLX_REQUIRE(C->ptr_width == 8 || nnz_c <= INT32_MAX, BCLX_ERR_OVERFLOW,
-                 "nnz(C) = " + std::to_string(nnz_c) + " does not fit an int32 row_ptr; request ptr_width 8");
+    BCLX_REQUIRE(C->m == tp.m, BCLX_ERR_INVALID, "C has the wrong number of rows");
+    BCLX_REQUIRE(tp.nnz_c == 0 || (C->col_idx && C->vals), BCLX_ERR_INVALID, "C->col_idx / C->vals are null");
+    BCLX_REQUIRE(C->ptr_width == 4 || C->ptr_width == 8, BCLX_ERR_INVALID, "C->ptr_width must be 4 or 8");
     BCLX_REQUIRE(C->dtype == (std::is_same<T, float>::value ? BCLX_F32 : BCLX_F64), BCLX_ERR_INVALID,
                  "C dtype must match A and B");
-    hipStream_t s = h->stream;
-    F.tiles[0].c_col = C->col_idx;
-    F.tiles[0].c_val = static_cast<T*>(C->vals);
-    F.R.out_limit = C->nnz;
-    F.upload_tiles();
-    F.bin(s, 0);
-    int32_t counts[kNumBins];
-    BCLX_HIP(hipMemcpyAsync(counts, F.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
-    BCLX_HIP(hipStreamSynchronize(s));
-    for (int b = 1; b < kNumBins; b++) F.run_bin(s, 1, b, counts[b]);
-    const unsigned g = div_up(m + 1, 256);
-    if (C->ptr_width == 8)
-      hipLaunchKernelGGL(k_rowptr<int64_t>, dim3(g), dim3(256), 0, s, F.R.off, (int64_t)0, m,
-                         static_cast<int64_t*>(C->row_ptr));
-    else
-      hipLaunchKernelGGL(k_rowptr<int32_t>, dim3(g), dim3(256), 0, s, F.R.off, (int64_t)0, m,
-                         static_cast<int32_t*>(C->row_ptr));
-    BCLX_HIP(hipGetLastError());
-    C->nnz = nnz_c;
+    tp.numeric(C->col_idx, static_cast<T*>(C->vals), C->nnz, C->row_ptr, C->ptr_width);
+    C->nnz = tp.nnz_c;
   }
   void rows(int64_t* prod, int32_t* nnz) override {
-    if (prod) BCLX_HIP(hipMemcpy(prod, F.R.prod, sizeof(int64_t) * m, hipMemcpyDeviceToHost));
-    if (nnz) BCLX_HIP(hipMemcpy(nnz, F.R.nnz, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+    if (prod) BCLX_HIP(hipMemcpy(prod, tp.F.R.prod, sizeof(int64_t) * tp.m, hipMemcpyDeviceToHost));
+    if (nnz) BCLX_HIP(hipMemcpy(nnz, tp.F.R.nnz, sizeof(int32_t) * tp.m, hipMemcpyDeviceToHost));
   }
 };
 
@@ -378,10 +418,10 @@ bclx_status bclx_spgemm_symbolic(bclx_handle* h, const bclx_csr* A, const bclx_c
     p->h = h;
     const bool f64 = A->dtype == BCLX_F64;
     const bool w8 = A->ptr_width == 8;
-    if (!f64 && !w8) p->impl = std::make_unique<TwoPhase<float, int32_t>>(h, A, B);
-    else if (!f64 && w8) p->impl = std::make_unique<TwoPhase<float, int64_t>>(h, A, B);
-    else if (f64 && !w8) p->impl = std::make_unique<TwoPhase<double, int32_t>>(h, A, B);
-    else p->impl = std::make_unique<TwoPhase<double, int64_t>>(h, A, B);
+    if (!f64 && !w8) p->impl = std::make_unique<TwoPhasePlan<float, int32_t>>(h, A, B);
+    else if (!f64 && w8) p->impl = std::make_unique<TwoPhasePlan<float, int64_t>>(h, A, B);
+    else if (f64 && !w8) p->impl = std::make_unique<TwoPhasePlan<double, int32_t>>(h, A, B);
+    else p->impl = std::make_unique<TwoPhasePlan<double, int64_t>>(h, A, B);
     *nnz_c = p->impl->symbolic();
     *plan = p.release();
   });

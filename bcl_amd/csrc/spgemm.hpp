@@ -40,6 +40,28 @@ struct FlatRows {
   void scan(hipStream_t s, const int32_t* cnt, int64_t* off, int64_t off_add);
   void run_bin(hipStream_t s, int mode, int b, int64_t count_hint);
   void compact(hipStream_t s, const int64_t* dst_off, int* dst_col, T* dst_val, const int* src_col, const T* src_val);
+  // heavy rows: allocate the zeroed accumulators of the heavy kernel (before any graph capture)
+  void prepare_heavy();
+  int64_t read_stat(int q);
 };
+
+// The two-phase product C = A * B of one tile pair (symbolic, then numeric into caller-allocated C).
+template <typename T, typename RP>
+struct TwoPhase {
+  bclx_handle* h;
+  FlatRows<T, RP> F;
+  int64_t nnz_c = -1;
+  int64_t m = 0;
+  TwoPhase(bclx_handle* h_, const TileDev<T, RP>& tile, const BinTuning& tn) : h(h_) {
+    std::vector<TileDev<T, RP>> tiles{tile};
+    F.setup(h, tiles, false);
+    F.R.tn = tn;
+    m = tile.m;
+  }
+  int64_t symbolic();
+  // writes col/val into c_col/c_val (capacity c_cap entries) and row_ptr (m+1 entries of 4 or 8 bytes)
+  void numeric(int* c_col, T* c_val, int64_t c_cap, void* row_ptr, int ptr_width);
+};
+
 
 }  // namespace bclx

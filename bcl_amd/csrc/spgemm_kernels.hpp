@@ -18,9 +18,10 @@ constexpr int kRowBlock = 256;         // flat-row padding and analysis block si
 constexpr int kHeavyThreads = 1024;
 
 // Table geometry of LDS bins 1..9: TS = 2^(b+4) slots holds up to TS/2 keys (load <= 1/2); G lanes per row.
+// Groups are either part of one wavefront (G <= 64, several rows per block) or the whole block (G >= 256).
 __host__ __device__ constexpr int bin_table(int b) { return 1 << (b + 4); }
 __host__ __device__ constexpr int bin_group(int b) {
-  return b == 1 ? 16 : b == 2 ? 32 : b <= 4 ? 64 : b == 5 ? 128 : b <= 7 ? 256 : b == 8 ? 512 : 1024;
+  return b == 1 ? 16 : b == 2 ? 32 : b <= 5 ? 64 : b == 6 ? 256 : b <= 8 ? 512 : 1024;
 }
 
 // Tunable binning knobs (chosen by the autotuner; these are the defaults).
@@ -54,7 +55,8 @@ struct TileDev {
   T* c_val;
   int64_t flat0;  // first flat row of this product
   int32_t m, n;
-  float avg_a, avg_b;  // average row lengths of A and B (predictor features)
+  float avg_a, avg_b;      // average row lengths of A and B (predictor features)
+  int64_t krows;           // rows of B
 };
 
 template <typename T, typename RP>
@@ -175,6 +177,10 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_
     if (tid == lt) prod = s;
   }
   R.prod[fr] = valid ? prod : 0;
+  {
+    const long long ub0 = prod < (long long)tile.n ? prod : (long long)tile.n;
+    if (valid && bin_for(ub0, prod, R.tn) == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
+  }
   if (do_bin) {
     const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
     const int b = valid ? bin_for(ub, prod, R.tn) : 0;
@@ -224,6 +230,94 @@ __global__ __launch_bounds__(kRowBlock) void k_bin(RowsDev<T, RP> R, int src) {
 // Values accumulate in float64 (products of fp32 inputs are exact in float64) and round once on output.
 // ------------------------------------------------------------------------------------------------------------
 
+// Synchronise the lanes of one row group: a wavefront-level fence when the group lives inside one wavefront, a
+// workgroup barrier when the group is the whole block.
+template <int G>
+__device__ inline void group_sync() {
+  if constexpr (G > 64) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+}
+
+// Exclusive scan of one value per lane over a row group; *total = group sum.
+template <int G, int BLOCK>
+__device__ inline long long group_excl_scan(long long v, long long* total, long long* smem) {
+  if constexpr (G > 64) {
+    static_assert(G == BLOCK, "multi-wavefront groups must be the whole block");
+    return block_excl_scan<BLOCK, long long>(v, total, smem);
+  } else {
+    const int gl = threadIdx.x & (G - 1);
+    long long inc = v;
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+      const long long t = __shfl_up(inc, d, G);
+      if (gl >= d) inc += t;
+    }
+    *total = __shfl(inc, G - 1, G);
+    return inc - v;
+  }
+}
+
+// Load-balanced enumeration of the products of one row: the group stages the B-row ranges of up to G entries of
+// A's row in LDS, prefix-sums their lengths, and lane l takes products l, l + G, ... (entry found by binary search),
+// two at a time so their loads overlap. Calls f(j, v) per product (column j, value a * b in float64; v = 0 when
+// !NUM). `stop` (optional, LDS) ends the enumeration early once set.
+template <int G, int BLOCK, bool NUM, typename T, typename RP, typename F>
+__device__ inline void for_each_product(const TileDev<T, RP>& tile, RP a0, RP a1, int lane, RP* e_start,
+                                        long long* e_pre, T* e_val, long long* scan_smem, const volatile int* stop,
+                                        F&& f) {
+  for (RP c0 = a0; c0 < a1; c0 += G) {
+    const int ne = (int)((a1 - c0) < (RP)G ? (a1 - c0) : (RP)G);
+    long long len = 0;
+    if (lane < ne) {
+      const int kk = tile.a_col[c0 + lane];
+      const RP st = tile.b_rp[kk];
+      len = (long long)(tile.b_rp[kk + 1] - st);
+      e_start[lane] = st;
+      if (NUM) e_val[lane] = tile.a_val[c0 + lane];
+    }
+    long long total;
+    const long long ex = group_excl_scan<G, BLOCK>(len, &total, scan_smem);
+    if (lane < ne) e_pre[lane] = ex;
+    if (lane == 0) e_pre[ne] = total;
+    group_sync<G>();
+    for (long long t = lane; t < total; t += 2 * G) {
+      if (stop && *stop) break;
+      const long long t2 = t + G;
+      const bool has2 = t2 < total;
+      int lo = 0, hi = ne;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (e_pre[mid] <= t) lo = mid; else hi = mid;
+      }
+      int lo2 = lo, hi2 = ne;
+      if (has2) {
+        while (hi2 - lo2 > 1) {
+          const int mid = (lo2 + hi2) >> 1;
+          if (e_pre[mid] <= t2) lo2 = mid; else hi2 = mid;
+        }
+      }
+      const RP q1 = e_start[lo] + (RP)(t - e_pre[lo]);
+      const RP q2 = has2 ? e_start[lo2] + (RP)(t2 - e_pre[lo2]) : q1;
+      const int j1 = tile.b_col[q1];
+      const int j2 = tile.b_col[q2];
+      double v1 = 0.0, v2 = 0.0;
+      if (NUM) {
+        const T bv1 = tile.b_val[q1], bv2 = tile.b_val[q2];
+        v1 = (double)e_val[lo] * (double)bv1;
+        v2 = (double)e_val[lo2] * (double)bv2;
+      }
+      f(j1, v1);
+      if (has2) f(j2, v2);
+    }
+    group_sync<G>();
+  }
+}
+
 template <int TS, int G, int BLOCK>
 __device__ inline void bitonic_sort_groups(int* keys, double* vals, int lane, bool with_vals) {
   // Sort each group's TS-slot table ascending by key; kEmptyKey sorts last. All threads of the block call this.
@@ -255,20 +349,64 @@ __device__ inline void bitonic_sort_groups(int* keys, double* vals, int lane, bo
   }
 }
 
+// Block-wide min / max of an int (BLOCK <= 1024); smem >= BLOCK/64 ints. Contains barriers.
+template <int BLOCK>
+__device__ inline int block_min(int v, int* smem) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+  if ((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = smem[0];
+  for (int q = 1; q < BLOCK / 64; q++) r = min(r, smem[q]);
+  __syncthreads();
+  return r;
+}
+template <int BLOCK>
+__device__ inline int block_max(int v, int* smem) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+  if ((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = smem[0];
+  for (int q = 1; q < BLOCK / 64; q++) r = max(r, smem[q]);
+  __syncthreads();
+  return r;
+}
+
+// LDS for the bitmap ranking of one row: W words of bits over the row's column span plus one prefix count per
+// 8 words. kRankWords words cover spans up to 2^19 columns.
+constexpr int kRankWords = 16384;
+constexpr int kRankBytes = kRankWords * 4 + (kRankWords / 8 + 16) * 4;
+
 template <int TS, int G, int BLOCK, int MODE, typename T, typename RP>
 __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
   constexpr int NG = BLOCK / G;
   constexpr bool NUM = MODE != 0;
-  __shared__ int s_keys[NG * TS];
-  __shared__ double s_vals[NUM ? NG * TS : 1];
+  // Whole-block rows with large tables: the sorted order comes from ranking every key in a bitmap of the row's
+  // column span (prefix popcounts) instead of sorting the table; entries go straight to their output position.
+  constexpr bool RANK = NUM && NG == 1 && TS >= 2048;
+  constexpr int TBYTES = NG * TS * (NUM ? 12 : 4);
+  // entry staging of for_each_product: start (RP), prefix (long long), A value (T) per lane, 16-byte aligned
+  constexpr int EPRE = ((NG * (G + 1) * 8) + 15) / 16 * 16;
+  constexpr int EST = ((NG * G * (int)sizeof(RP)) + 15) / 16 * 16;
+  constexpr int EBYTES = EPRE + EST + ((NG * G * (int)sizeof(T)) + 15) / 16 * 16;
+  // the staging lives after the table; the rank bitmap later reuses the whole buffer
+  constexpr int LBYTES = (RANK && kRankBytes > TBYTES + EBYTES) ? kRankBytes : TBYTES + EBYTES;
+  __shared__ __align__(16) unsigned char s_buf[LBYTES];
   __shared__ int s_cnt[NG];
   __shared__ int s_flag[NG];
   __shared__ long long s_base[NG];
+  __shared__ int s_red[BLOCK / 64 + 1];
+  __shared__ long long s_lscan[BLOCK / 64 + 1];
+  long long* s_epre = reinterpret_cast<long long*>(s_buf + TBYTES);
+  RP* s_est = reinterpret_cast<RP*>(s_buf + TBYTES + EPRE);
+  T* s_eval = reinterpret_cast<T*>(s_buf + TBYTES + EPRE + EST);
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
-  int* keys = s_keys + g * TS;
-  double* vals = s_vals + (NUM ? g * TS : 0);
+  int* keys = reinterpret_cast<int*>(s_buf) + g * TS;
+  double* vals = reinterpret_cast<double*>(s_buf + NG * TS * 4) + (NUM ? g * TS : 0);
   const int count = R.bin_count[bin];
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
+  volatile int* vflag = s_flag;
 
   for (int base = blockIdx.x * NG; base < count; base += gridDim.x * NG) {
     const int idx = base + g;
@@ -289,47 +427,34 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
       const int64_t i = fr - tile.flat0;
       const int limit = MODE == 2 ? R.cap[fr] : INT_MAX;
       const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
-      const long long prod = R.prod[fr];
-      const long long na = (long long)(a1 - a0);
-      const long long avg = (prod + na - 1) / (na > 0 ? na : 1);
-      int lg = 0;
-      while ((1 << lg) < avg && (1 << lg) < G) lg++;
-      const int L = 1 << lg;
-      const int S = G >> lg;
-      const int sub = lane >> lg, sl = lane & (L - 1);
-      for (RP p = a0 + sub; p < a1; p += S) {
-        if (MODE == 2 && s_flag[g]) break;
-        const int kk = tile.a_col[p];
-        const double av = NUM ? (double)tile.a_val[p] : 0.0;
-        const RP b0 = tile.b_rp[kk], b1 = tile.b_rp[kk + 1];
-        for (RP q = b0 + sl; q < b1; q += L) {
-          const int j = tile.b_col[q];
-          const double pv = NUM ? av * (double)tile.b_val[q] : 0.0;
-          unsigned h = ((unsigned)j * 107u) & (TS - 1);
-          for (int probe = 0; probe < TS; probe++) {
-            const int kcur = keys[h];
-            if (kcur == j) {
-              if (NUM) atomicAdd(&vals[h], pv);
-              break;
-            }
-            if (kcur == kEmptyKey) {
-              const int old = atomicCAS(&keys[h], kEmptyKey, j);
-              if (old == kEmptyKey) {
-                const int c = atomicAdd(&s_cnt[g], 1);
-                if (MODE == 2 && c >= limit) s_flag[g] = 1;
-                if (NUM) atomicAdd(&vals[h], pv);
-                break;
-              }
-              if (old == j) {
-                if (NUM) atomicAdd(&vals[h], pv);
-                break;
-              }
-            }
-            h = (h + 1) & (TS - 1);
-            if (MODE == 2 && probe == TS - 1) s_flag[g] = 1;  // table full: only possible after overflow
+      auto insert = [&](int j, double pv) {
+        unsigned h = ((unsigned)j * 107u) & (TS - 1);
+        for (int probe = 0; probe < TS; probe++) {
+          const int kcur = keys[h];
+          if (kcur == j) {
+            if (NUM) atomicAdd(&vals[h], pv);
+            return;
           }
+          if (kcur == kEmptyKey) {
+            const int old = atomicCAS(&keys[h], kEmptyKey, j);
+            if (old == kEmptyKey) {
+              const int c = atomicAdd(&s_cnt[g], 1);
+              if (MODE == 2 && c >= limit) vflag[g] = 1;
+              if (NUM) atomicAdd(&vals[h], pv);
+              return;
+            }
+            if (old == j) {
+              if (NUM) atomicAdd(&vals[h], pv);
+              return;
+            }
+          }
+          h = (h + 1) & (TS - 1);
         }
-      }
+        if (MODE == 2) vflag[g] = 1;  // table full: only possible after overflow
+      };
+      for_each_product<G, BLOCK, NUM, T, RP>(tile, a0, a1, lane, s_est + g * G, s_epre + g * (G + 1),
+                                              s_eval + (NUM ? g * G : 0), s_lscan, MODE == 2 ? vflag + g : nullptr,
+                                              insert);
     }
     __syncthreads();
     if (MODE == 0) {
@@ -352,18 +477,198 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
           R.nnz[fr] = cnt;
         }
       }
-      bitonic_sort_groups<TS, G, BLOCK>(keys, vals, lane, true);
-      __syncthreads();
-      const long long o = MODE == 3 ? s_base[g] : (active ? R.off[fr] : 0);
-      const bool fits = o + cnt <= R.out_limit;
-      if (active && !ovf && !fits && lane == 0) R.stats[3] = 1ull;
-      if (active && !ovf && fits) {
-        for (int s = lane; s < cnt; s += G) {
-          tile.c_col[o + s] = keys[s];
-          tile.c_val[o + s] = (T)vals[s];
+      bool ranked = false;
+      if constexpr (RANK) {
+        constexpr int SPT = TS / BLOCK;
+        int rk[SPT];
+        double rv[SPT];
+        int kmin = INT_MAX, kmax = -1;
+#pragma unroll
+        for (int q = 0; q < SPT; q++) {
+          rk[q] = keys[q * BLOCK + threadIdx.x];
+          rv[q] = vals[q * BLOCK + threadIdx.x];
+          if (rk[q] != kEmptyKey) {
+            kmin = min(kmin, rk[q]);
+            kmax = max(kmax, rk[q]);
+          }
+        }
+        kmin = block_min<BLOCK>(kmin, s_red);
+        kmax = block_max<BLOCK>(kmax, s_red);
+        const long long span = kmax >= kmin ? (long long)kmax - kmin + 1 : 1;
+        const int W = (int)((span + 31) / 32);
+        if (W <= kRankWords && (W * 4 + ((W + 7) / 8 + 1) * 4) <= LBYTES) {
+          ranked = true;
+          unsigned* bits = reinterpret_cast<unsigned*>(s_buf);
+          int* pre = reinterpret_cast<int*>(s_buf + (size_t)W * 4);
+          const int NGR = (W + 7) / 8;
+          for (int w = threadIdx.x; w < W; w += BLOCK) bits[w] = 0u;
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < SPT; q++)
+            if (rk[q] != kEmptyKey) {
+              const int rel = rk[q] - kmin;
+              atomicOr(&bits[rel >> 5], 1u << (rel & 31));
+            }
+          __syncthreads();
+          const int gpt = (NGR + BLOCK - 1) / BLOCK;
+          const int g0 = threadIdx.x * gpt, g1 = min(NGR, g0 + gpt);
+          int c = 0;
+          for (int gg = g0; gg < g1; gg++)
+            for (int w = gg * 8; w < min(gg * 8 + 8, W); w++) c += __popc(bits[w]);
+          int tot;
+          int run = block_excl_scan<BLOCK, int>(c, &tot, s_red);
+          for (int gg = g0; gg < g1; gg++) {
+            pre[gg] = run;
+            for (int w = gg * 8; w < min(gg * 8 + 8, W); w++) run += __popc(bits[w]);
+          }
+          __syncthreads();
+          const long long o = MODE == 3 ? s_base[0] : (active ? R.off[fr] : 0);
+          const bool fits = o + cnt <= R.out_limit;
+          if (active && !ovf && !fits && threadIdx.x == 0) R.stats[3] = 1ull;
+          if (active && !ovf && fits) {
+#pragma unroll
+            for (int q = 0; q < SPT; q++) {
+              if (rk[q] == kEmptyKey) continue;
+              const int rel = rk[q] - kmin;
+              const int w = rel >> 5;
+              int r = pre[w >> 3];
+              for (int w2 = (w >> 3) * 8; w2 < w; w2++) r += __popc(bits[w2]);
+              r += __popc(bits[w] & ((1u << (rel & 31)) - 1u));
+              tile.c_col[o + r] = rk[q];
+              tile.c_val[o + r] = (T)rv[q];
+            }
+          }
+        } else {
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < SPT; q++) {
+            keys[q * BLOCK + threadIdx.x] = rk[q];
+            vals[q * BLOCK + threadIdx.x] = rv[q];
+          }
+          __syncthreads();
+        }
+      }
+      if (!ranked) {
+        bitonic_sort_groups<TS, G, BLOCK>(keys, vals, lane, true);
+        __syncthreads();
+        const long long o = MODE == 3 ? s_base[g] : (active ? R.off[fr] : 0);
+        const bool fits = o + cnt <= R.out_limit;
+        if (active && !ovf && !fits && lane == 0) R.stats[3] = 1ull;
+        if (active && !ovf && fits) {
+          for (int s = lane; s < cnt; s += G) {
+            tile.c_col[o + s] = keys[s];
+            tile.c_val[o + s] = (T)vals[s];
+          }
         }
       }
     }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Heavy rows (more output entries than the largest LDS table), rank-compact accumulator: one workgroup per row.
+//   pass 1: LDS bitmap of the row's columns (one atomicOr per product) -> per-word prefix popcounts, so every
+//           column's output rank is known and nnz(C_i) is exact;
+//   pass 2 (numeric modes): each product is added, in float64, into a per-workgroup scratch indexed by rank, i.e. a
+//           compact array of nnz(C_i) doubles that stays in L2 (a column-indexed dense array would be n doubles);
+//   emission walks the bitmap in column order and writes (column, value) at the row's offset + rank.
+// Modes as in k_hash: 0 symbolic, 1 numeric at off[fr], 2 predicted (slot if nnz <= cap, else bump-allocate),
+// 3 bump-allocate. Needs n <= 32 * kHRWords; wider tiles use k_heavy (global dense accumulator).
+// ------------------------------------------------------------------------------------------------------------
+
+constexpr int kHRBlock = 1024;
+constexpr int kHRWords = 16384;
+
+template <int MODE, typename T, typename RP>
+__global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int bin, double* scratch,
+                                                         int64_t scratch_stride) {
+  __shared__ unsigned s_bits[kHRWords];
+  __shared__ int s_pre[kHRWords];
+  __shared__ long long s_scan[kHRBlock / 64 + 1];
+  __shared__ long long s_base;
+  __shared__ RP s_est[kHRBlock];
+  __shared__ long long s_epre[kHRBlock + 1];
+  __shared__ T s_eval[kHRBlock];
+  __shared__ long long s_lscan[kHRBlock / 64 + 1];
+  constexpr bool NUM = MODE != 0;
+  const int tid = threadIdx.x;
+  double* acc = scratch + (int64_t)blockIdx.x * scratch_stride;
+  const int count = R.bin_count[bin];
+  const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
+  for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
+    const int fr = items[idx];
+    const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
+    const int64_t i = fr - tile.flat0;
+    const int W = (tile.n + 31) / 32;
+    for (int w = tid; w < W; w += kHRBlock) s_bits[w] = 0u;
+    __syncthreads();
+    const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
+    for_each_product<kHRBlock, kHRBlock, false, T, RP>(
+        tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
+        [&](int j, double) { atomicOr(&s_bits[j >> 5], 1u << (j & 31)); });
+    __syncthreads();
+    const int per = (W + kHRBlock - 1) / kHRBlock;
+    const int w0 = tid * per, w1 = min(W, w0 + per);
+    long long c = 0;
+    for (int w = w0; w < w1; w++) c += __popc(s_bits[w]);
+    long long total;
+    long long run = block_excl_scan<kHRBlock, long long>(c, &total, s_scan);
+    for (int w = w0; w < w1; w++) {
+      s_pre[w] = (int)run;
+      run += __popc(s_bits[w]);
+    }
+    if (MODE == 0) {
+      if (tid == 0) {
+        R.nnz[fr] = (int)total;
+        atomicAdd(&R.stats[1], 1ull);
+      }
+      __syncthreads();
+      continue;
+    }
+    if (tid == 0) {
+      long long b;
+      if (MODE == 1 || (MODE == 2 && total <= (long long)R.cap[fr])) {
+        b = R.off[fr];
+      } else {
+        b = (long long)atomicAdd(R.cursor, (unsigned long long)total);
+        R.off[fr] = b;
+      }
+      if (b + total > R.out_limit) {
+        R.stats[3] = 1ull;
+        b = -1;
+        if (MODE != 1) R.off[fr] = -1;
+      }
+      s_base = b;
+      R.nnz[fr] = (int)total;
+      atomicAdd(&R.stats[1], 1ull);
+    }
+    __syncthreads();
+    for_each_product<kHRBlock, kHRBlock, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
+                                                       [&](int j, double v) {
+                                                         const int w = j >> 5;
+                                                         const int r = s_pre[w] + __popc(s_bits[w] & ((1u << (j & 31)) - 1u));
+                                                         unsafeAtomicAdd(&acc[r], v);
+                                                       });
+    __threadfence();
+    __syncthreads();
+    const long long base = s_base;
+    for (int w = w0; w < w1; w++) {
+      unsigned word = s_bits[w];
+      int r = s_pre[w];
+      while (word) {
+        const int bit = __ffs(word) - 1;
+        const double v = __hip_atomic_load(&acc[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (base >= 0) {
+          tile.c_col[base + r] = w * 32 + bit;
+          tile.c_val[base + r] = (T)v;
+        }
+        acc[r] = 0.0;
+        r++;
+        word &= word - 1;
+      }
+    }
+    __threadfence();
     __syncthreads();
   }
 }

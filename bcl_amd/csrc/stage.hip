@@ -91,7 +91,7 @@ struct StagePlan : StageBase {
   bclx_stage_opts opts{};
   BinTuning tn;
   // mode 0
-  std::vector<std::unique_ptr<FlatRows<T, RP>>> per;  // one workspace per product
+  std::vector<std::unique_ptr<TwoPhase<T, RP>>> per;  // one two-phase workspace per product
   std::vector<DevBuf> c_rp, c_col, c_val;
   std::vector<int64_t> c_nnz;
   // modes 1 / 2
@@ -143,37 +143,11 @@ struct StagePlan : StageBase {
   }
 
   // ---- mode 0 ----------------------------------------------------------------------------------------------
-  void plain_product(int p, bool allocate) {
-    FlatRows<T, RP>& W = *per[p];
-    hipStream_t s = h->stream;
-    W.reset_counters(s);
-    W.analyze(s, true);
-    int32_t counts[kNumBins];
-    BCLX_HIP(hipMemcpyAsync(counts, W.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
-    BCLX_HIP(hipStreamSynchronize(s));
-    for (int b = 1; b < kNumBins; b++) W.run_bin(s, 0, b, counts[b]);
-    W.scan(s, W.R.nnz, W.R.off, 0);
-    int64_t tot = 0;
-    BCLX_HIP(hipMemcpyAsync(&tot, W.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    BCLX_HIP(hipStreamSynchronize(s));
-    if (allocate) {
-      c_nnz[p] = tot;
-      c_col[p].alloc(sizeof(int) * std::max<int64_t>(tot, 1));
-      c_val[p].alloc(sizeof(T) * std::max<int64_t>(tot, 1));
-      c_rp[p].alloc(sizeof(int64_t) * (W.tiles[0].m + 1));
-      W.tiles[0].c_col = c_col[p].as<int>();
-      W.tiles[0].c_val = c_val[p].as<T>();
-      W.upload_tiles();
-    }
+  void plain_product(int p) {
+    TwoPhase<T, RP>& W = *per[p];
+    const int64_t tot = W.symbolic();
     BCLX_REQUIRE(tot <= c_nnz[p], BCLX_ERR_OVERFLOW, "stage product grew since the plan was made");
-    W.R.out_limit = c_nnz[p];
-    W.bin(s, 0);
-    BCLX_HIP(hipMemcpyAsync(counts, W.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
-    BCLX_HIP(hipStreamSynchronize(s));
-    for (int b = 1; b < kNumBins; b++) W.run_bin(s, 1, b, counts[b]);
-    hipLaunchKernelGGL(k_rowptr<int64_t>, dim3(div_up(W.tiles[0].m + 1, 256)), dim3(256), 0, s, W.R.off, (int64_t)0,
-                       (int64_t)W.tiles[0].m, c_rp[p].as<int64_t>());
-    BCLX_HIP(hipGetLastError());
+    W.numeric(c_col[p].as<int>(), c_val[p].as<T>(), c_nnz[p], c_rp[p].p, 8);
   }
 
   void setup_plain(const std::vector<TileDev<T, RP>>& tiles) {
@@ -182,11 +156,13 @@ struct StagePlan : StageBase {
     c_val.resize(npairs);
     c_nnz.assign(npairs, 0);
     for (int p = 0; p < npairs; p++) {
-      per.push_back(std::make_unique<FlatRows<T, RP>>());
-      per.back()->setup(h, {tiles[p]}, false);
-      per.back()->R.tn = tn;
+      per.push_back(std::make_unique<TwoPhase<T, RP>>(h, tiles[p], tn));
+      const int64_t tot = per.back()->symbolic();  // sizes the outputs
+      c_nnz[p] = tot;
+      c_col[p].alloc(sizeof(int) * std::max<int64_t>(tot, 1));
+      c_val[p].alloc(sizeof(T) * std::max<int64_t>(tot, 1));
+      c_rp[p].alloc(sizeof(int64_t) * (tiles[p].m + 1));
     }
-    for (int p = 0; p < npairs; p++) plain_product(p, true);  // sizes outputs
     BCLX_HIP(hipStreamSynchronize(h->stream));
   }
 
@@ -247,6 +223,14 @@ struct StagePlan : StageBase {
   }
 
   unsigned counts_ovf = 0;
+  bool has_heavy = false;
+
+  // after the sizing run's analysis: window indexes and scratch for the rows that can take the heavy path
+  void setup_heavy(hipStream_t s) {
+    (void)s;
+    has_heavy = F.read_stat(5) > 0;
+    if (has_heavy) F.prepare_heavy();
+  }
 
   void read_counts(int32_t* dst) {
     BCLX_HIP(hipMemcpyAsync(dst, F.R.bin_count, sizeof(int32_t) * kNumBins, hipMemcpyDeviceToHost, h->stream));
@@ -267,6 +251,7 @@ struct StagePlan : StageBase {
     if (mode == 1) {
       fill_i32(s, F.R.nnz, F.nflat, 0);
       F.analyze(s, false);
+      setup_heavy(s);
       hipLaunchKernelGGL((k_predict<T, RP>), dim3((unsigned)F.nblocks), dim3(kRowBlock), 0, s, F.R,
                          params.as<float>(), opts.margin, opts.slack, opts.h_mlp_params ? 1 : 0);
       BCLX_HIP(hipGetLastError());
@@ -309,7 +294,7 @@ struct StagePlan : StageBase {
         BCLX_HIP(hipMemcpyAsync(&cur, F.R.cursor, sizeof(cur), hipMemcpyDeviceToHost, s));
         BCLX_HIP(hipStreamSynchronize(s));
         // the cursor counts every request, granted or refused, so cur - main_cap is what this input needs
-        ok = read_stat(3) == 0;
+        ok = F.read_stat(3) == 0;
         if (!ok) reserve = ((int64_t)cur - main_cap) + ((int64_t)cur - main_cap) / 16 + 4096;
       }
       BCLX_REQUIRE(ok, BCLX_ERR_OVERFLOW, "could not size the overflow region of the stage");
@@ -322,6 +307,7 @@ struct StagePlan : StageBase {
       final_val.alloc(sizeof(T) * final_cap);
     } else {
       F.analyze(s, true);
+      setup_heavy(s);
       read_counts(counts1);
       for (int b = 1; b < kNumBins; b++) F.run_bin(s, 0, b, counts1[b]);
       F.scan(s, F.R.nnz, F.R.off, 0);
@@ -374,7 +360,7 @@ struct StagePlan : StageBase {
 
   void execute() override {
     if (mode == 0) {
-      for (int p = 0; p < npairs; p++) plain_product(p, false);
+      for (int p = 0; p < npairs; p++) plain_product(p);
       return;
     }
     if (exec) BCLX_HIP(hipGraphLaunch(exec, h->stream));
@@ -387,8 +373,8 @@ struct StagePlan : StageBase {
     C->ptr_width = 8;
     C->dtype = std::is_same<T, float>::value ? BCLX_F32 : BCLX_F64;
     if (mode == 0) {
-      C->m = per[p]->tiles[0].m;
-      C->n = per[p]->tiles[0].n;
+      C->m = per[p]->F.tiles[0].m;
+      C->n = per[p]->F.tiles[0].n;
       C->nnz = c_nnz[p];
       C->row_ptr = c_rp[p].p;
       C->col_idx = c_col[p].as<int>();
@@ -415,11 +401,11 @@ struct StagePlan : StageBase {
     if (mode == 0) {
       for (int p = 0; p < npairs; p++) {
         unsigned long long v[2];
-        BCLX_HIP(hipMemcpy(v, per[p]->R.stats, sizeof(v), hipMemcpyDeviceToHost));
+        BCLX_HIP(hipMemcpy(v, per[p]->F.R.stats, sizeof(v), hipMemcpyDeviceToHost));
         st[0] += (int64_t)v[0];
         st[6] += (int64_t)v[1];
         st[1] += c_nnz[p];
-        rows += per[p]->tiles[0].m;
+        rows += per[p]->F.tiles[0].m;
       }
       st[4] = 0;
     } else {
