@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-fvisibility=hidden", "-I" + os.path.join(ROOT, "include"), "-Wno-unused-result",
 ]
-HIP_SOURCES = ["core.hip", "spgemm.hip", "stage.hip", "mlp.hip"]
+HIP_SOURCES = ["core.hip", "spgemm.hip", "stage.hip", "mlp.hip", "spmm.hip", "gemm.hip", "accumulate.hip"]
 CPP_SOURCES = ["stubs.cpp"]
 
 
@@ -69,6 +69,17 @@ def build_gen() -> str:
     return out
 
 
+def build_cpp_driver() -> str:
+    """The C++ host-layer test driver (tests/cpp), linked against libbclx.so and the HIP runtime."""
+    src = os.path.join(ROOT, "tests", "cpp", "host_layer_driver.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "host_layer_driver")
+    _run(["g++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),
+          "-I/opt/rocm/include", src, "-o", out + ".tmp", "-L" + LIBDIR, "-lbclx", "-L/opt/rocm/lib", "-lamdhip64",
+          "-Wl,-rpath," + LIBDIR, "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle() -> None:
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "-j4"])
 
@@ -77,6 +88,7 @@ def build_all(jobs: int = 8) -> None:
     build_gen()
     build_oracle()
     build_bclx(jobs)
+    build_cpp_driver()
 
 
 if __name__ == "__main__":

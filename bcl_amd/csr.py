@@ -67,6 +67,26 @@ class CSR:
         return CSR(m, n, np.zeros(m + 1, np.int64), np.zeros(0, np.int32), np.zeros(0, dtype))
 
 
+def write_binary(a: CSR, path: str, index_dtype=np.int32) -> None:
+    """BCL's binary CSR file (CSRMatrix::write_Binary, bcl/containers/sequential/CSRMatrix.hpp:513-530): m, n, nnz
+    as size_t, then vals, col_ind, row_ptr (row_ptr and col_ind in the index type)."""
+    with open(path, "wb") as f:
+        np.array([a.m, a.n, a.nnz], np.uint64).tofile(f)
+        np.ascontiguousarray(a.vals).tofile(f)
+        np.ascontiguousarray(a.col_idx, index_dtype).tofile(f)
+        np.ascontiguousarray(a.row_ptr, index_dtype).tofile(f)
+
+
+def read_binary(path: str, val_dtype=np.float32, index_dtype=np.int32) -> CSR:
+    """Read BCL's binary CSR file (CSRMatrix::read_Binary, CSRMatrix.hpp:488-511)."""
+    with open(path, "rb") as f:
+        m, n, nnz = (int(x) for x in np.fromfile(f, np.uint64, 3))
+        vals = np.fromfile(f, val_dtype, nnz)
+        col = np.fromfile(f, index_dtype, nnz).astype(np.int32)
+        rp = np.fromfile(f, index_dtype, m + 1).astype(np.int64)
+    return CSR(m, n, rp, col, vals)
+
+
 def grid_tile_shape(m: int, n: int, grid_m: int, grid_n: int):
     """BlockCustom with Tile::div (bcl/containers/detail/Blocking.hpp:223-232): ceil(m/grid_m), ceil(n/grid_n)."""
     return (m + grid_m - 1) // grid_m, (n + grid_n - 1) // grid_n

@@ -1,0 +1,321 @@
+// Dense GEMM C = beta * C + alpha * op(A) * op(B), row-major (bclx_gemm): the tile multiply BCL's dense gemm
+// calls through cblas_gemm_wrapper_ (bcl/containers/algorithms/cblas_wrapper.hpp:16-40, called with
+// alpha = beta = 1 from gemm_cowns, bcl/containers/algorithms/gemm.hpp:86-107).
+//
+// Three arithmetic modes on the matrix cores:
+//   0  FP32: fp32 inputs, v_mfma_f32_32x32x2f32 (exact fp32 products, fp32 accumulation)
+//   1  BF16: inputs rounded to bf16 (round to nearest even), v_mfma_f32_32x32x16_bf16, fp32 accumulation
+//   2  split-BF16 3-term: x = x_hi + x_lo with x_hi = bf16(x), x_lo = bf16(x - x_hi); a*b is taken as
+//      a_lo*b_hi + a_hi*b_lo + a_hi*b_hi (the a_lo*b_lo term, below 2^-16 relative, is dropped), fp32 accumulation
+//
+// Pipeline: a pack pass writes op(A) as Ap[M'][K'] and op(B) as Bp[N'][K'] (K contiguous, zero-padded to the block
+// tile, converted / split for the bf16 modes), then the MFMA kernel runs on the packed operands with no bounds checks
+// in its main loop. Block tile 128 x 128, four wavefronts of 64 x 64 (2 x 2 MFMA tiles of 32 x 32), the next K tile
+// is loaded into registers while the current one is multiplied out of LDS. Blocks are ordered so that each XCD
+// (blocks are dealt round-robin over the 8 XCDs) works on a contiguous band of C tiles and shares their A panels in
+// its L2.
+#include <algorithm>
+#include <type_traits>
+
+#include "internal.hpp"
+
+namespace bclx {
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E), so register arrays get constant indices.
+template <int B, int E, typename F>
+__device__ inline void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGBM = 128, kGBN = 128, kGThreads = 256;
+
+__device__ inline uint16_t f32_to_bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);  // NaN stays a (quiet) NaN
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_bits_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// dst[r * ldd + c] for r < R', c < K' from src[r * rs + c * cs] (r < R, c < K), zero padding outside.
+// MODE 0: fp32 copy; 1: bf16 (hi only); 2: bf16 hi and lo. 32 x 32 tiles staged through LDS so that both the
+// read (along the source's contiguous direction) and the write (along c) are coalesced.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ src, int64_t R, int64_t K, int64_t rs,
+                                              int64_t cs, void* dst_hi, uint16_t* dst_lo, int64_t Rp, int64_t Kp) {
+  __shared__ float tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  const bool rows_contig = cs != 1;  // source contiguous along r (a transposed operand)
+  for (int q = ty; q < 32; q += 8) {
+    int64_t r, c;
+    if (rows_contig) { r = r0 + tx; c = c0 + q; }
+    else { r = r0 + q; c = c0 + tx; }
+    const float v = (r < R && c < K) ? src[r * rs + c * cs] : 0.f;
+    if (rows_contig) tile[tx][q] = v;
+    else tile[q][tx] = v;
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += 8) {
+    const int64_t r = r0 + q, c = c0 + tx;
+    if (r >= Rp || c >= Kp) continue;
+    const float v = tile[q][tx];
+    if (MODE == 0) {
+      static_cast<float*>(dst_hi)[r * Kp + c] = v;
+    } else {
+      const uint16_t hi = f32_to_bf16_rne(v);
+      static_cast<uint16_t*>(dst_hi)[r * Kp + c] = hi;
+      if (MODE == 2) dst_lo[r * Kp + c] = f32_to_bf16_rne(v - bf16_bits_to_f32(hi));
+    }
+  }
+}
+
+template <int MODE>
+struct GemmCfg;
+template <>
+struct GemmCfg<0> {
+  using E = float;
+  static constexpr int BK = 32, PAD = 4, KSTEP = 2;
+};
+template <>
+struct GemmCfg<1> {
+  using E = uint16_t;
+  static constexpr int BK = 64, PAD = 8, KSTEP = 16;
+};
+template <>
+struct GemmCfg<2> {
+  using E = uint16_t;
+  static constexpr int BK = 64, PAD = 8, KSTEP = 16;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kGThreads) void k_gemm(const typename GemmCfg<MODE>::E* __restrict__ Ahi,
+                                                    const typename GemmCfg<MODE>::E* __restrict__ Alo,
+                                                    const typename GemmCfg<MODE>::E* __restrict__ Bhi,
+                                                    const typename GemmCfg<MODE>::E* __restrict__ Blo, int64_t M,
+                                                    int64_t N, int64_t Kp, float alpha, float beta, float* C,
+                                                    int64_t ldc, int tiles_m, int tiles_n) {
+  using E = typename GemmCfg<MODE>::E;
+  constexpr int BK = GemmCfg<MODE>::BK, PAD = GemmCfg<MODE>::PAD, KSTEP = GemmCfg<MODE>::KSTEP;
+  constexpr int LDK = BK + PAD;
+  constexpr int NSPLIT = MODE == 2 ? 2 : 1;
+  constexpr int VPR = BK * (int)sizeof(E) / 16;      // 16-byte vectors per tile row
+  constexpr int VPT = kGBM * VPR / kGThreads;        // vectors per thread per operand
+  __shared__ __align__(16) E As[NSPLIT][kGBM * LDK];
+  __shared__ __align__(16) E Bs[NSPLIT][kGBN * LDK];
+
+  // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous run of tiles (row-major over C tiles)
+  const int T = tiles_m * tiles_n;
+  int b = blockIdx.x;
+  if (T % 8 == 0) b = (b % 8) * (T / 8) + (b / 8);
+  const int tm = b / tiles_n, tn = b % tiles_n;
+  const int64_t m0 = (int64_t)tm * kGBM, n0 = (int64_t)tn * kGBN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  const E* __restrict__ Ag0 = Ahi + m0 * Kp;
+  const E* __restrict__ Bg0 = Bhi + n0 * Kp;
+  const E* __restrict__ Ag1 = MODE == 2 ? Alo + m0 * Kp : Ag0;
+  const E* __restrict__ Bg1 = MODE == 2 ? Blo + n0 * Kp : Bg0;
+  constexpr int EPV = 16 / (int)sizeof(E);  // elements per 16-byte vector
+  // per-thread vector slots: row = (tid + i * 256) / VPR, column vector = (tid + i * 256) % VPR
+  u32x4 ra0[VPT], rb0[VPT], ra1[VPT], rb1[VPT];
+  auto gload = [&](int64_t k0) {
+    static_for<0, VPT>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int v = tid + i * kGThreads;
+      const int64_t o = (int64_t)(v / VPR) * Kp + k0 + (v % VPR) * EPV;
+      ra0[i] = *reinterpret_cast<const u32x4*>(Ag0 + o);
+      rb0[i] = *reinterpret_cast<const u32x4*>(Bg0 + o);
+      if constexpr (MODE == 2) {
+        ra1[i] = *reinterpret_cast<const u32x4*>(Ag1 + o);
+        rb1[i] = *reinterpret_cast<const u32x4*>(Bg1 + o);
+      }
+    });
+  };
+  auto lstore = [&]() {
+    static_for<0, VPT>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int v = tid + i * kGThreads;
+      const int o = (v / VPR) * LDK + (v % VPR) * EPV;
+      *reinterpret_cast<u32x4*>(&As[0][o]) = ra0[i];
+      *reinterpret_cast<u32x4*>(&Bs[0][o]) = rb0[i];
+      if constexpr (MODE == 2) {
+        *reinterpret_cast<u32x4*>(&As[NSPLIT - 1][o]) = ra1[i];
+        *reinterpret_cast<u32x4*>(&Bs[NSPLIT - 1][o]) = rb1[i];
+      }
+    });
+  };
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[a][c][v] = 0.f;
+
+  const int nk = (int)(Kp / BK);
+  gload(0);
+  lstore();
+  __syncthreads();
+  const int fr = lane & 31, fk = lane >> 5;
+  for (int kt = 0; kt < nk; kt++) {
+    if (kt + 1 < nk) gload((int64_t)(kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += KSTEP) {
+      if constexpr (MODE == 0) {
+        float af[2], bf[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          af[t] = As[0][(wm * 64 + t * 32 + fr) * LDK + ks + fk];
+          bf[t] = Bs[0][(wn * 64 + t * 32 + fr) * LDK + ks + fk];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int c = 0; c < 2; c++) acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[a], bf[c], acc[a][c], 0, 0, 0);
+      } else {
+        bf16x8 ah[2], bh[2], al[2], bl[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          ah[t] = *reinterpret_cast<const bf16x8*>(&As[0][(wm * 64 + t * 32 + fr) * LDK + ks + fk * 8]);
+          bh[t] = *reinterpret_cast<const bf16x8*>(&Bs[0][(wn * 64 + t * 32 + fr) * LDK + ks + fk * 8]);
+          if constexpr (MODE == 2) {
+            al[t] = *reinterpret_cast<const bf16x8*>(&As[NSPLIT - 1][(wm * 64 + t * 32 + fr) * LDK + ks + fk * 8]);
+            bl[t] = *reinterpret_cast<const bf16x8*>(&Bs[NSPLIT - 1][(wn * 64 + t * 32 + fr) * LDK + ks + fk * 8]);
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int c = 0; c < 2; c++) {
+            if constexpr (MODE == 2) {
+              acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
+              acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
+            }
+            acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
+          }
+      }
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      lstore();
+      __syncthreads();
+    }
+  }
+  // accumulator layout of a 32 x 32 MFMA tile: lane l holds column l % 32, rows 8 * (v / 4) + 4 * (l / 32) + v % 4
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int64_t col = n0 + wn * 64 + c * 32 + fr;
+      if (col >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int64_t row = m0 + wm * 64 + a * 32 + 8 * (v / 4) + 4 * fk + (v % 4);
+        if (row < M) {
+          float* p = C + row * ldc + col;
+          *p = beta == 0.f ? alpha * acc[a][c][v] : fmaf(beta, *p, alpha * acc[a][c][v]);
+        }
+      }
+    }
+}
+
+// FP64 reference: C += A * B (fp32 inputs widened exactly), 16 x 16 tiles, row-major. Error oracle only.
+__global__ __launch_bounds__(256) void k_gemm_f64(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
+                                                  const float* B, int64_t ldb, double* C, int64_t ldc) {
+  __shared__ double as[16][17], bs[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int64_t row = (int64_t)blockIdx.y * 16 + ty, col = (int64_t)blockIdx.x * 16 + tx;
+  double s = 0.0;
+  for (int64_t k0 = 0; k0 < K; k0 += 16) {
+    as[ty][tx] = (row < M && k0 + tx < K) ? (double)A[row * lda + k0 + tx] : 0.0;
+    bs[ty][tx] = (k0 + ty < K && col < N) ? (double)B[(k0 + ty) * ldb + col] : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) s = fma(as[ty][q], bs[q][tx], s);
+    __syncthreads();
+  }
+  if (row < M && col < N) C[row * ldc + col] += s;
+}
+
+template <int MODE>
+static void gemm_impl(bclx_handle* h, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const float* A,
+                      int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc) {
+  using E = typename GemmCfg<MODE>::E;
+  constexpr int BK = GemmCfg<MODE>::BK;
+  const int64_t Mp = (m + kGBM - 1) / kGBM * kGBM, Np = (n + kGBN - 1) / kGBN * kGBN;
+  const int64_t Kp = std::max<int64_t>(BK, (k + BK - 1) / BK * BK);
+  const size_t a_bytes = (size_t)Mp * Kp * sizeof(E), b_bytes = (size_t)Np * Kp * sizeof(E);
+  const size_t lo_a = MODE == 2 ? (size_t)Mp * Kp * 2 : 0, lo_b = MODE == 2 ? (size_t)Np * Kp * 2 : 0;
+  const size_t off_b = (a_bytes + 255) / 256 * 256, off_la = off_b + (b_bytes + 255) / 256 * 256;
+  const size_t off_lb = off_la + (lo_a + 255) / 256 * 256, total = off_lb + lo_b + 256;
+  h->scratch.reserve(total);
+  char* ws = h->scratch.as<char>();
+  E* Ap = reinterpret_cast<E*>(ws);
+  E* Bp = reinterpret_cast<E*>(ws + off_b);
+  uint16_t* Al = MODE == 2 ? reinterpret_cast<uint16_t*>(ws + off_la) : nullptr;
+  uint16_t* Bl = MODE == 2 ? reinterpret_cast<uint16_t*>(ws + off_lb) : nullptr;
+  // op(A)[i][kk] = ta ? A[kk * lda + i] : A[i * lda + kk]; Bp[j][kk] = op(B)[kk][j] = tb ? B[j * ldb + kk] : B[kk * ldb + j]
+  dim3 blk(256);
+  hipLaunchKernelGGL(k_pack<MODE>, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), blk, 0, h->stream, A, m, k,
+                     ta ? (int64_t)1 : lda, ta ? lda : (int64_t)1, (void*)Ap, Al, Mp, Kp);
+  BCLX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pack<MODE>, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), blk, 0, h->stream, B, n, k,
+                     tb ? ldb : (int64_t)1, tb ? (int64_t)1 : ldb, (void*)Bp, Bl, Np, Kp);
+  BCLX_HIP(hipGetLastError());
+  const int tiles_m = (int)(Mp / kGBM), tiles_n = (int)(Np / kGBN);
+  hipLaunchKernelGGL(k_gemm<MODE>, dim3((unsigned)(tiles_m * tiles_n)), dim3(kGThreads), 0, h->stream, Ap, (const E*)Al,
+                     Bp, (const E*)Bl, m, n, Kp, alpha, beta, C, ldc, tiles_m, tiles_n);
+  BCLX_HIP(hipGetLastError());
+}
+
+}  // namespace bclx
+
+using namespace bclx;
+
+extern "C" {
+
+bclx_status bclx_gemm(bclx_handle* h, int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k, float alpha,
+                      const float* d_A, int64_t lda, const float* d_B, int64_t ldb, float beta, float* d_C,
+                      int64_t ldc, int32_t mode) {
+  return guarded([&] {
+    BCLX_REQUIRE(h && d_C, BCLX_ERR_INVALID, "null handle or C");
+    BCLX_REQUIRE(m >= 0 && n >= 0 && k >= 0, BCLX_ERR_INVALID, "negative size");
+    BCLX_REQUIRE(mode >= 0 && mode <= 2, BCLX_ERR_INVALID, "mode must be 0 (fp32), 1 (bf16) or 2 (split-bf16)");
+    BCLX_REQUIRE(ldc >= n, BCLX_ERR_INVALID, "ldc < n");
+    BCLX_REQUIRE(lda >= (trans_a ? m : k) && ldb >= (trans_b ? k : n), BCLX_ERR_INVALID, "lda / ldb too small");
+    BCLX_REQUIRE(k == 0 || (d_A && d_B), BCLX_ERR_INVALID, "null A or B");
+    set_device(h);
+    if (m == 0 || n == 0) return;  // k == 0 packs zeros, so C = beta * C
+    const bool ta = trans_a != 0, tb = trans_b != 0;
+    switch (mode) {
+      case 0: gemm_impl<0>(h, ta, tb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc); break;
+      case 1: gemm_impl<1>(h, ta, tb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc); break;
+      default: gemm_impl<2>(h, ta, tb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc); break;
+    }
+  });
+}
+
+bclx_status bclx_gemm_f64_ref(bclx_handle* h, int64_t m, int64_t n, int64_t k, const float* d_A, int64_t lda,
+                              const float* d_B, int64_t ldb, double* d_C, int64_t ldc) {
+  return guarded([&] {
+    BCLX_REQUIRE(h && d_A && d_B && d_C && m >= 0 && n >= 0 && k >= 0, BCLX_ERR_INVALID, "bad argument");
+    set_device(h);
+    if (m == 0 || n == 0) return;
+    hipLaunchKernelGGL(k_gemm_f64, dim3(div_up(n, 16), div_up(m, 16)), dim3(256), 0, h->stream, m, n, k, d_A, lda, d_B,
+                       ldb, d_C, ldc);
+    BCLX_HIP(hipGetLastError());
+  });
+}
+
+}  // extern "C"

@@ -172,6 +172,22 @@ static void launch_lds_bin(const RowsDev<T, RP>& R, hipStream_t s, int b, unsign
   }
 }
 
+int64_t scan_partials(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+void scan_counts(hipStream_t s, const int32_t* cnt, int64_t n, int64_t* off, long long* partial, int64_t* total) {
+  if (n == 0) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned long long*>(off), (int64_t)1, 0ull);
+    if (total) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned long long*>(total), (int64_t)1, 0ull);
+    BCLX_HIP(hipGetLastError());
+    return;
+  }
+  const int64_t nb = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanBlock), 0, s, cnt, n, partial);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, partial, nb, total);
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kScanBlock), 0, s, cnt, n, partial, off, (int64_t)0);
+  BCLX_HIP(hipGetLastError());
+}
+
 int rows_per_block(int b) {
   const int G = bin_group(b);
   const int BLOCK = G < 256 ? 256 : G;

@@ -12,6 +12,10 @@ namespace bclx {
 void fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
 void fill_u64(hipStream_t s, unsigned long long* p, int64_t n, unsigned long long v);
 int rows_per_block(int bin);
+// off[0..n] = exclusive scan of cnt[0..n) (off[n] = total, also written to *total when non-null);
+// partial needs scan_partials(n) entries.
+int64_t scan_partials(int64_t n);
+void scan_counts(hipStream_t s, const int32_t* cnt, int64_t n, int64_t* off, long long* partial, int64_t* total);
 void check_pair(const bclx_csr* A, const bclx_csr* B);
 
 template <typename T, typename RP>

@@ -43,6 +43,15 @@ __host__ __device__ inline int bin_for(int64_t need, int64_t work, const BinTuni
   return bw;
 }
 
+// Home slot of column j in a table of TS = 2^b slots: Fibonacci hashing (top b bits of j * 2^32 / phi), so every
+// bit of j moves the slot. (A multiplier-and-mask hash keeps only j's low bits; R-MAT columns have biased bits, so
+// their low bits take few distinct values and linear probing would chain.)
+template <int TS>
+__device__ inline unsigned slot_hash(int j) {
+  constexpr int b = __builtin_ctz(TS);
+  return ((unsigned)j * 0x9E3779B1u) >> (32 - b);
+}
+
 template <typename T, typename RP>
 struct TileDev {
   const RP* a_rp;
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
       const int limit = MODE == 2 ? R.cap[fr] : INT_MAX;
       const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
       auto insert = [&](int j, double pv) {
-        unsigned h = ((unsigned)j * 107u) & (TS - 1);
+        unsigned h = slot_hash<TS>(j);
         for (int probe = 0; probe < TS; probe++) {
           const int kcur = keys[h];
           if (kcur == j) {
@@ -840,6 +849,9 @@ static __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int32_t*
 template <typename T, typename RP>
 __global__ __launch_bounds__(kRowBlock) void k_compact(RowsDev<T, RP> R, const int64_t* dst_off, int* dst_col,
                                                        T* dst_val, const int* src_col, const T* src_val) {
+  // Copy each row's entries from the arena (at off[fr], nnz[fr] long) to the final CSR (at dst_off[fr]). A block
+  // takes 256 flat rows; its output range is contiguous, and each thread copies 4 consecutive outputs per step (one
+  // binary search for the first, then it walks row boundaries), so loads and stores stay coalesced and independent.
   __shared__ long long s_dst[kRowBlock + 1];
   __shared__ long long s_src[kRowBlock];
   const int tid = threadIdx.x;
@@ -849,15 +861,34 @@ __global__ __launch_bounds__(kRowBlock) void k_compact(RowsDev<T, RP> R, const i
   s_src[tid] = R.off[fr0 + tid];
   __syncthreads();
   const long long lo = s_dst[0], hi = s_dst[kRowBlock];
-  for (long long e = lo + tid; e < hi; e += kRowBlock) {
-    int a = 0, b = kRowBlock;  // find r with s_dst[r] <= e < s_dst[r+1]
+  for (long long e0 = lo + 4LL * tid; e0 < hi; e0 += 4LL * kRowBlock) {
+    int a = 0, b = kRowBlock;  // row r with s_dst[r] <= e0 < s_dst[r + 1]
     while (b - a > 1) {
-      int mid = (a + b) >> 1;
-      if (s_dst[mid] <= e) a = mid; else b = mid;
+      const int mid = (a + b) >> 1;
+      if (s_dst[mid] <= e0) a = mid; else b = mid;
     }
-    const long long src = s_src[a] + (e - s_dst[a]);
-    dst_col[e] = src_col[src];
-    dst_val[e] = src_val[src];
+    int cv[4];
+    T vv[4];
+    long long src[4];
+    int r = a;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const long long e = e0 + q;
+      while (r < kRowBlock - 1 && s_dst[r + 1] <= e) r++;
+      src[q] = s_src[r] + (e - s_dst[r]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (e0 + q < hi) {
+        cv[q] = src_col[src[q]];
+        vv[q] = src_val[src[q]];
+      }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (e0 + q < hi) {
+        dst_col[e0 + q] = cv[q];
+        dst_val[e0 + q] = vv[q];
+      }
   }
 }
 

@@ -1,0 +1,190 @@
+// SpMM: C[i*ldc + j] = beta * C + alpha * sum_k A[i,k] * B[k*ldb + j], j < n (bclx_spmm).
+//
+// The reference's spmm_wrapper (bcl/containers/algorithms/spgemm.hpp:78-96, :140-149) is alpha = beta = 1 with
+// row-major B and C and explicit ldb / ldc (DMatrix tiles padded to the full tile width).
+//
+// Work split: A's nonzeros are cut into chunks of kChunk consecutive entries; one wavefront takes a chunk, finds the
+// row holding its first entry by binary search in row_ptr, and walks the chunk's entries in order. Each lane owns
+// VEC consecutive columns of a 64*VEC column pass; for every entry the wavefront gathers one B row segment
+// (coalesced), multiplies by the entry's value (broadcast from the lane that loaded it) and accumulates in fp32.
+// A row that lies wholly inside the chunk is written with one read-modify-write of its C segment; a row cut by a
+// chunk boundary is added with fp32 atomics (each chunk adds its own part). B is fp32 or bf16 (converted on load);
+// accumulation is fp32.
+#include <algorithm>
+
+#include "internal.hpp"
+
+namespace bclx {
+
+constexpr int kSpmmChunk = 256;  // entries per wavefront
+constexpr int kSpmmWaves = 4;    // wavefronts per block
+
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+template <typename BT, int VEC>
+struct BLoad;
+template <>
+struct BLoad<float, 2> {
+  static __device__ inline void load(const float* p, float* o) {
+    const float2 v = *reinterpret_cast<const float2*>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+  }
+};
+template <>
+struct BLoad<uint16_t, 2> {
+  static __device__ inline void load(const uint16_t* p, float* o) {
+    const uint32_t v = *reinterpret_cast<const uint32_t*>(p);
+    o[0] = __uint_as_float(v << 16);
+    o[1] = __uint_as_float(v & 0xFFFF0000u);
+  }
+};
+template <typename BT>
+struct BLoad<BT, 1> {
+  static __device__ inline void load(const BT* p, float* o) {
+    if constexpr (sizeof(BT) == 2) o[0] = bf16_to_f32(*p);
+    else o[0] = *p;
+  }
+};
+
+template <typename RP, typename BT, int VEC>
+__global__ __launch_bounds__(64 * kSpmmWaves) void k_spmm(const RP* __restrict__ rp, const int* __restrict__ col,
+                                                          const float* __restrict__ val, int64_t m, int64_t nnz,
+                                                          const BT* __restrict__ B, int64_t ldb, float* C, int64_t ldc,
+                                                          int n, float alpha) {
+  const int lane = threadIdx.x & 63;
+  const int64_t chunk = (int64_t)blockIdx.x * kSpmmWaves + (threadIdx.x >> 6);
+  const int64_t e0 = chunk * kSpmmChunk;
+  if (e0 >= nnz) return;
+  const int64_t e1 = e0 + kSpmmChunk < nnz ? e0 + kSpmmChunk : nnz;
+  // first row with rp[r + 1] > e0
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)rp[mid + 1] <= e0) lo = mid + 1;
+    else hi = mid;
+  }
+  for (int c0 = 0; c0 < n; c0 += 64 * VEC) {
+    const int cj = c0 + lane * VEC;
+    const bool cok = cj < n;
+    int64_t r = lo;
+    int64_t rs = (int64_t)rp[r], re = (int64_t)rp[r + 1];
+    float acc[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; v++) acc[v] = 0.f;
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      // stage 64 entries: lane q holds entry eb + q
+      const int64_t eq = eb + lane;
+      const int my_col = eq < e1 ? col[eq] : 0;
+      const float my_val = eq < e1 ? val[eq] : 0.f;
+      const int cnt = (int)((e1 - eb) < 64 ? (e1 - eb) : 64);
+      for (int q = 0; q < cnt; q++) {
+        const int64_t e = eb + q;
+        while (e >= re) {  // flush the finished row (empty rows have nothing to add), move to the row holding e
+          if (cok && re > rs) {
+            float* cr = C + r * ldc + cj;
+            const bool whole = rs >= e0;  // row started inside this chunk and ends here
+#pragma unroll
+            for (int v = 0; v < VEC; v++) {
+              if (cj + v >= n) break;
+              if (whole) cr[v] += alpha * acc[v];
+              else atomicAdd(&cr[v], alpha * acc[v]);
+            }
+          }
+#pragma unroll
+          for (int v = 0; v < VEC; v++) acc[v] = 0.f;
+          r++;
+          rs = re;
+          re = (int64_t)rp[r + 1];
+        }
+        const int k = __shfl(my_col, q, 64);
+        const float a = __shfl(my_val, q, 64);
+        if (cok) {
+          float b[VEC];
+          if (cj + VEC <= n) {
+            BLoad<BT, VEC>::load(B + (int64_t)k * ldb + cj, b);
+          } else {
+#pragma unroll
+            for (int v = 0; v < VEC; v++) b[v] = 0.f;
+            for (int v = 0; v < VEC && cj + v < n; v++) BLoad<BT, 1>::load(B + (int64_t)k * ldb + cj + v, &b[v]);
+          }
+#pragma unroll
+          for (int v = 0; v < VEC; v++) acc[v] = fmaf(a, b[v], acc[v]);
+        }
+      }
+    }
+    // last row of the chunk: whole only if it started inside the chunk and ends at e1
+    if (cok) {
+      float* cr = C + r * ldc + cj;
+      const bool whole = rs >= e0 && re <= e1;
+#pragma unroll
+      for (int v = 0; v < VEC; v++) {
+        if (cj + v >= n) break;
+        if (whole) cr[v] += alpha * acc[v];
+        else atomicAdd(&cr[v], alpha * acc[v]);
+      }
+    }
+  }
+}
+
+__global__ void k_scale_rows(float* C, int64_t m, int64_t n, int64_t ldc, float beta) {
+  const int64_t total = m * n;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / n, j = t % n;
+    float* p = C + i * ldc + j;
+    *p = beta == 0.f ? 0.f : beta * *p;
+  }
+}
+
+template <typename RP, typename BT>
+static void spmm_impl(bclx_handle* h, const bclx_csr* A, const void* Bv, int64_t ldb, float* C, int64_t ldc,
+                      int64_t n, float alpha) {
+  const RP* rp = static_cast<const RP*>(A->row_ptr);
+  const BT* B = static_cast<const BT*>(Bv);
+  const int64_t chunks = (A->nnz + kSpmmChunk - 1) / kSpmmChunk;
+  if (chunks == 0 || n == 0) return;
+  const unsigned grid = div_up(chunks, kSpmmWaves);
+  const bool vec2 = (n % 2 == 0) && (ldb % 2 == 0) && (ldc % 2 == 0) &&
+                    ((reinterpret_cast<uintptr_t>(B) % (2 * sizeof(BT))) == 0);
+  if (vec2)
+    hipLaunchKernelGGL((k_spmm<RP, BT, 2>), dim3(grid), dim3(64 * kSpmmWaves), 0, h->stream, rp, A->col_idx,
+                       static_cast<const float*>(A->vals), A->m, A->nnz, B, ldb, C, ldc, (int)n, alpha);
+  else
+    hipLaunchKernelGGL((k_spmm<RP, BT, 1>), dim3(grid), dim3(64 * kSpmmWaves), 0, h->stream, rp, A->col_idx,
+                       static_cast<const float*>(A->vals), A->m, A->nnz, B, ldb, C, ldc, (int)n, alpha);
+  BCLX_HIP(hipGetLastError());
+}
+
+}  // namespace bclx
+
+using namespace bclx;
+
+extern "C" {
+
+bclx_status bclx_spmm(bclx_handle* h, const bclx_csr* A, const void* d_B, int32_t b_dtype, int64_t ldb, float* d_C,
+                      int64_t ldc, int64_t n, float alpha, float beta) {
+  return guarded([&] {
+    BCLX_REQUIRE(h && A && d_C, BCLX_ERR_INVALID, "null argument");
+    BCLX_REQUIRE(A->dtype == BCLX_F32, BCLX_ERR_INVALID, "A must hold fp32 values");
+    BCLX_REQUIRE(A->ptr_width == 4 || A->ptr_width == 8, BCLX_ERR_INVALID, "A->ptr_width must be 4 or 8");
+    BCLX_REQUIRE(b_dtype == BCLX_F32 || b_dtype == BCLX_BF16, BCLX_ERR_INVALID, "B must be fp32 or bf16");
+    BCLX_REQUIRE(n >= 0 && ldb >= n && ldc >= n, BCLX_ERR_INVALID, "need ldb >= n and ldc >= n");
+    BCLX_REQUIRE(A->nnz == 0 || (A->col_idx && A->vals && d_B), BCLX_ERR_INVALID, "null A arrays or B");
+    set_device(h);
+    if (beta != 1.f && A->m > 0 && n > 0) {
+      const int64_t total = A->m * n;
+      hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 65536)), dim3(256), 0,
+                         h->stream, d_C, A->m, n, ldc, beta);
+      BCLX_HIP(hipGetLastError());
+    }
+    if (A->ptr_width == 8) {
+      if (b_dtype == BCLX_F32) spmm_impl<int64_t, float>(h, A, d_B, ldb, d_C, ldc, n, alpha);
+      else spmm_impl<int64_t, uint16_t>(h, A, d_B, ldb, d_C, ldc, n, alpha);
+    } else {
+      if (b_dtype == BCLX_F32) spmm_impl<int32_t, float>(h, A, d_B, ldb, d_C, ldc, n, alpha);
+      else spmm_impl<int32_t, uint16_t>(h, A, d_B, ldb, d_C, ldc, n, alpha);
+    }
+  });
+}
+
+}  // extern "C"

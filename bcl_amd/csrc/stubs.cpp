@@ -14,12 +14,5 @@ void set_error(const std::string& msg);
 
 extern "C" {
 
-bclx_status bclx_accumulate_tiles(bclx_handle*, int, const bclx_csr*, bclx_csr*) { BCLX_STUB("bclx_accumulate_tiles"); }
-bclx_status bclx_spmm(bclx_handle*, const bclx_csr*, const void*, int32_t, int64_t, float*, int64_t, int64_t, float,
-                      float) { BCLX_STUB("bclx_spmm"); }
-bclx_status bclx_gemm(bclx_handle*, int32_t, int32_t, int64_t, int64_t, int64_t, float, const float*, int64_t,
-                      const float*, int64_t, float, float*, int64_t, int32_t) { BCLX_STUB("bclx_gemm"); }
-bclx_status bclx_gemm_f64_ref(bclx_handle*, int64_t, int64_t, int64_t, const float*, int64_t, const float*, int64_t,
-                              double*, int64_t) { BCLX_STUB("bclx_gemm_f64_ref"); }
 
 }  // extern "C"

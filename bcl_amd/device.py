@@ -110,8 +110,70 @@ def spgemm(h: Handle, A: DeviceCSR, B: DeviceCSR, ptr_dtype=torch.int64, return_
     return (C, rows) if return_rows else C
 
 
+def spmm(h: Handle, A: DeviceCSR, B: torch.Tensor, C: torch.Tensor, n: int | None = None, alpha: float = 1.0,
+         beta: float = 1.0) -> torch.Tensor:
+    """C = beta * C + alpha * A @ B[:, :n] in place (BCL's spmm_wrapper is alpha = beta = 1). B row-major fp32 or
+    bf16 (k x ldb), C row-major fp32 (m x ldc)."""
+    assert B.dim() == 2 and C.dim() == 2 and B.stride(1) == 1 and C.stride(1) == 1
+    assert C.dtype == torch.float32 and B.dtype in (torch.float32, torch.bfloat16)
+    n = B.shape[1] if n is None else n
+    d = A.desc()
+    bt = _lib.BCLX_F32 if B.dtype == torch.float32 else _lib.BCLX_BF16
+    call("bclx_spmm", h.h, ctypes.byref(d), _ptr(B), bt, B.stride(0), _ptr(C), C.stride(0), n, alpha, beta)
+    return C
+
+
+def accumulate(h: Handle, tiles: list, ptr_dtype=torch.int64) -> DeviceCSR:
+    """C = sum of partial tiles (SparseHashAccumulator::get_matrix on the GPU), two calls through the C ABI."""
+    arr = (bclx_csr * len(tiles))(*[t.desc() for t in tiles])
+    dev = tiles[0].vals.device
+    C = DeviceCSR(tiles[0].m, tiles[0].n, torch.empty(tiles[0].m + 1, dtype=ptr_dtype, device=dev),
+                  torch.empty(0, dtype=torch.int32, device=dev), torch.empty(0, dtype=tiles[0].vals.dtype, device=dev))
+    d = C.desc()
+    d.col_idx = None
+    d.vals = None
+    call("bclx_accumulate_tiles", h.h, len(tiles), arr, ctypes.byref(d))
+    nnz = d.nnz
+    if nnz == 0:
+        return C  # the first call already wrote row_ptr
+    C.col_idx = torch.empty(nnz, dtype=torch.int32, device=dev)
+    C.vals = torch.empty(nnz, dtype=tiles[0].vals.dtype, device=dev)
+    d = C.desc()
+    d.nnz = nnz
+    call("bclx_accumulate_tiles", h.h, len(tiles), arr, ctypes.byref(d))
+    return C
+
+
+GEMM_FP32, GEMM_BF16, GEMM_SPLIT_BF16 = 0, 1, 2
+
+
+def gemm(h: Handle, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, mode: int = GEMM_FP32, trans_a: bool = False,
+         trans_b: bool = False, alpha: float = 1.0, beta: float = 1.0) -> torch.Tensor:
+    """C = beta * C + alpha * op(A) @ op(B) in place, row-major fp32 (cblas_gemm_wrapper_ with RowMajor).
+    mode: GEMM_FP32 (FP32 MFMA), GEMM_BF16 (bf16 inputs, fp32 accumulation), GEMM_SPLIT_BF16 (3-term split)."""
+    for t in (A, B, C):
+        assert t.dim() == 2 and (t.stride(1) == 1 or t.shape[1] <= 1) and t.dtype == torch.float32
+    m, k = (A.shape[1], A.shape[0]) if trans_a else (A.shape[0], A.shape[1])
+    kb, n = (B.shape[1], B.shape[0]) if trans_b else (B.shape[0], B.shape[1])
+    assert k == kb and C.shape[0] >= m and C.shape[1] >= n
+    lda = max(A.stride(0), A.shape[1])
+    ldb = max(B.stride(0), B.shape[1])
+    call("bclx_gemm", h.h, int(trans_a), int(trans_b), m, n, k, alpha, _ptr(A), lda, _ptr(B), ldb, beta, _ptr(C),
+         C.stride(0), mode)
+    return C
+
+
+def gemm_f64_ref(h: Handle, A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    """float64 A @ B on the GPU (fp32 inputs widened exactly): the error oracle for large dense products."""
+    m, k = A.shape
+    n = B.shape[1]
+    C = torch.zeros((m, n), dtype=torch.float64, device=A.device)
+    call("bclx_gemm_f64_ref", h.h, m, n, k, _ptr(A), A.stride(0), _ptr(B), B.stride(0), _ptr(C), C.stride(0))
+    return C
+
+
 def version() -> str:
     return lib().bclx_version().decode()
 
 
-__all__ = ["Handle", "DeviceCSR", "spgemm", "version", "_lib"]
+__all__ = ["Handle", "DeviceCSR", "spgemm", "spmm", "accumulate", "gemm", "gemm_f64_ref", "version", "_lib"]

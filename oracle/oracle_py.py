@@ -82,6 +82,8 @@ def ref():
         R.ref_result_copy_free.argtypes = [_P, ctypes.c_int, _P, _P, _P]
         R.ref_result_copy_free.restype = None
         R.ref_spmm_f32.argtypes = [_I64, _I64, _I64, _I64, _P, _P, _P, _P, _P, _I64, _I64]
+        R.ref_binary_roundtrip.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        R.ref_binary_roundtrip.restype = _I64
         R.ref_spmm_f32.restype = None
         _ref = R
     return _ref

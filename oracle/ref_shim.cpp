@@ -102,6 +102,14 @@ REF_API void ref_result_copy_free(void* r, int is_f64, int* row_ptr, int* col_in
   }
 }
 
+// CSRMatrix::read_Binary then write_Binary (bcl/containers/sequential/CSRMatrix.hpp:488-530) on a float/int file.
+REF_API int64_t ref_binary_roundtrip(const char* in, const char* out) {
+  BCL::CSRMatrix<float, int> a(0, 0);
+  a.read_Binary(in);
+  a.write_Binary(out);
+  return (int64_t)a.nnz_;
+}
+
 REF_API void ref_spmm_f32(int64_t m, int64_t n, int64_t k, int64_t nnz, float* c, float* b, float* a_val, int* a_rp,
                   int* a_col, int64_t ldb, int64_t ldc) {
   BCL::spmm_wrapper(m, n, k, nnz, c, b, a_val, a_rp, a_col, ldb, ldc);

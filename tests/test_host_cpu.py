@@ -112,3 +112,23 @@ def test_split_grid_slices(grid):
 def test_stage_pairs_cover_grid():
     from bcl_amd.csr import stage_pairs
     assert len(stage_pairs(8, 3)) == 64
+
+
+def test_binary_format_matches_reference(tmp_path):
+    """BCL's binary CSR file: written here, read and rewritten by the reference's CSRMatrix::read_Binary /
+    write_Binary, read back here unchanged."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    from bcl_amd.csr import read_binary, write_binary
+    a = gen.rmat(9, seed=4)
+    p1, p2 = str(tmp_path / "a.bin"), str(tmp_path / "b.bin")
+    write_binary(a, p1)
+    b = read_binary(p1)
+    np.testing.assert_array_equal(b.row_ptr, a.row_ptr)
+    np.testing.assert_array_equal(b.col_idx, a.col_idx)
+    np.testing.assert_array_equal(b.vals, a.vals)
+    if not O.have_ref():
+        pytest.skip("reference build not present")
+    assert O.ref().ref_binary_roundtrip(p1.encode(), p2.encode()) == a.nnz
+    assert open(p1, "rb").read() == open(p2, "rb").read()
