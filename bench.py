@@ -207,12 +207,201 @@ def bench_stage(args):
     return out
 
 
+def _ref_or_oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    return O
+
+
+def bench_config1(args):
+    """Plain CSR SpGEMM C = A * A on one 16384^2 tile, exactly 8 uniform nnz/row (two-phase C ABI per product)."""
+    import torch
+    from bcl_amd import gen
+    from bcl_amd.device import DeviceCSR, Handle, spgemm
+    O = _ref_or_oracle()
+    a = gen.uniform(16384, 16384, 8, seed=0xB0C1)
+    h = Handle(0)
+    A = DeviceCSR.from_host(a)
+    C = spgemm(h, A, A)
+    ms, per = time_steps(lambda: spgemm(h, A, A), args.steps, args.warmup, h.stream)
+    ms_step = ms / args.steps
+    products = int(np.diff(a.row_ptr)[a.col_idx].sum())
+    nnz_c = C.nnz
+    alg = 2 * csr_bytes(a.m, a.nnz) + (a.m + 1) * 8 + nnz_c * 8
+    gbs = alg / (ms_step * 1e-3) / 1e9
+    t = time.perf_counter()
+    ref = O.ref_spgemm(a, a) if O.have_ref() else O.spgemm(a, a, dead_row_alloc=True)
+    dt = time.perf_counter() - t
+    assert ref.nnz == nnz_c
+    return {"metric": "SpGEMM useful GFLOP/s, one 16384^2 tile, 8 nnz/row (two-phase, host-synchronised call)",
+            "value": round(2 * products / (ms_step * 1e-3) / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32 values, fp64 accumulate, int32 indices",
+            "data": "synthetic",
+            "config": {"workload": "uniform-16384-8perrow-AxA", "products": products, "nnz_c": nnz_c},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "algorithmic_bytes": alg,
+                         "kernel": "whole two-phase call incl. host synchronisation (latency bound at this size)"},
+            "cpu_baseline": {"value": round(2 * products / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
+                             "kind": "reference" if O.have_ref() else "port",
+                             "sample": f"the full product, {dt:.2f} s serial BCL::gemm(CSRMatrix)"}}
+
+
+def bench_spmm(args):
+    """SpMM C += A * B: R-MAT scale 20 (ef 16) times dense 2^20 x 128 (N(0,1)), fp32 B and bf16 B."""
+    import torch
+    from bcl_amd import gen
+    from bcl_amd.csr import CSR
+    from bcl_amd.device import DeviceCSR, Handle, spmm
+    O = _ref_or_oracle()
+    a = gen.rmat(args.spmm_scale, ef=16, seed=args.seed)
+    n = 128
+    h = Handle(0)
+    A = DeviceCSR.from_host(a)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    B = torch.randn((a.n, n), device="cuda", dtype=torch.float32, generator=g)
+    out = {}
+    rows = np.random.default_rng(0).choice(a.m, 2048, replace=False)
+    rows.sort()
+    sub = CSR(len(rows), a.n, np.concatenate([[0], np.cumsum(np.diff(a.row_ptr)[rows])]).astype(np.int64),
+              np.concatenate([a.col_idx[a.row_ptr[r]:a.row_ptr[r + 1]] for r in rows]),
+              np.concatenate([a.vals[a.row_ptr[r]:a.row_ptr[r + 1]] for r in rows]))
+    for name, Bt in (("fp32", B), ("bf16", B.to(torch.bfloat16))):
+        C = torch.zeros((a.m, n), device="cuda", dtype=torch.float32)
+        ms, _ = time_steps(lambda: spmm(h, A, Bt, C), args.steps, args.warmup, h.stream)
+        ms_step = ms / args.steps
+        C.zero_()
+        spmm(h, A, Bt, C)
+        torch.cuda.synchronize()
+        bh = B.cpu().numpy()
+        ref = O.spmm_f64(sub, bh)
+        got = C.cpu().numpy()[rows]
+        err = float(np.abs(got - ref).max() / np.abs(ref).max())
+        b_bytes = a.n * n * (4 if name == "fp32" else 2)
+        alg = csr_bytes(a.m, a.nnz) + b_bytes + 2 * a.m * n * 4
+        out[name] = {"ms": round(ms_step, 4), "gflops": round(2 * a.nnz * n / (ms_step * 1e-3) / 1e9, 2),
+                     "hbm_gbs": round(alg / (ms_step * 1e-3) / 1e9, 2), "max_err_vs_fp64_rel_to_max": err,
+                     "algorithmic_bytes": alg}
+    # reference CPU spmm_wrapper on a row sample
+    rs = min(a.m, 65536)
+    sub2 = CSR(rs, a.n, a.row_ptr[:rs + 1].copy(), a.col_idx[:a.row_ptr[rs]].copy(), a.vals[:a.row_ptr[rs]].copy())
+    bh = B.cpu().numpy()
+    cc = np.zeros((rs, n), np.float32)
+    t = time.perf_counter()
+    (O.ref_spmm if O.have_ref() else O.spmm)(sub2, bh, cc)
+    dt = time.perf_counter() - t
+    f = out["fp32"]
+    return {"metric": "SpMM GFLOP/s + HBM GB/s %peak (C += A*B, R-MAT A, dense B n=128)", "value": f["gflops"],
+            "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": f["ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (B fp32 or bf16)",
+            "data": "synthetic", "config": {"workload": f"rmat-s{args.spmm_scale}-ef16-x-dense128", "nnz_a": a.nnz},
+            "roofline": {"bound": "hbm", "achieved": f["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(f["hbm_gbs"] / HBM_PEAK_GBS, 5), "traffic": None,
+                         "algorithmic_bytes": f["algorithmic_bytes"], "kernel": "k_spmm (fp32 B)"},
+            "fp32_b": out["fp32"], "bf16_b": out["bf16"],
+            "cpu_baseline": {"value": round(2 * sub2.nnz * n / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
+                             "kind": "reference" if O.have_ref() else "port",
+                             "sample": f"rows 0..{rs - 1} ({sub2.nnz} nnz), serial spmm_wrapper, {dt:.1f} s"}}
+
+
+def bench_gemm(args):
+    """Dense tile GEMM 8192^3, N(0,1): FP32 MFMA, BF16 MFMA, split-BF16 3-term; error vs FP64 for each."""
+    import torch
+    from bcl_amd.device import GEMM_BF16, GEMM_FP32, GEMM_SPLIT_BF16, Handle, gemm, gemm_f64_ref
+    O = _ref_or_oracle()
+    N = args.gemm_n
+    h = Handle(0)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    A = torch.randn((N, N), device="cuda", generator=g)
+    B = torch.randn((N, N), device="cuda", generator=g)
+    C64 = gemm_f64_ref(h, A, B)
+    rows = np.arange(0, N, max(1, N // 16))[:16]
+    ref_rows = O.gemm_f64(A[rows].cpu().numpy(), B.cpu().numpy())
+    assert np.allclose(C64[rows].cpu().numpy(), ref_rows, rtol=1e-12, atol=1e-9), "FP64 GPU reference disagrees"
+    peak = {GEMM_FP32: 157.3, GEMM_BF16: 2500.0, GEMM_SPLIT_BF16: 2500.0}
+    issued = {GEMM_FP32: 1, GEMM_BF16: 1, GEMM_SPLIT_BF16: 3}
+    res = {}
+    for mode, name in ((GEMM_FP32, "fp32"), (GEMM_BF16, "bf16"), (GEMM_SPLIT_BF16, "split_bf16_3")):
+        C = torch.zeros((N, N), device="cuda")
+        ms, _ = time_steps(lambda: gemm(h, A, B, C, mode=mode, beta=0.0), args.steps, args.warmup, h.stream)
+        ms_step = ms / args.steps
+        torch.cuda.synchronize()
+        err = float(((C.double() - C64).abs().max() / C64.abs().max()).item())
+        useful = 2.0 * N ** 3 / (ms_step * 1e-3) / 1e12
+        res[name] = {"ms": round(ms_step, 3), "useful_tflops": round(useful, 2),
+                     "mfma_util": round(useful * issued[mode] / peak[mode], 4),
+                     "max_abs_err_rel_to_max_c": err}
+    f = res["bf16"]
+    # CPU: the oracle's float64 GEMM on 16 rows (labelled)
+    t = time.perf_counter()
+    O.gemm_f64(A[rows].cpu().numpy(), B.cpu().numpy(), nthreads=min(os.cpu_count() or 1, 16))
+    dt = time.perf_counter() - t
+    return {"metric": "Dense GEMM TFLOP/s and MFMA utilisation (8192^3)", "value": f["useful_tflops"],
+            "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": f["ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 in, fp32 accumulate",
+            "data": "synthetic", "config": {"workload": f"dense-gemm-{N}^3"},
+            "roofline": {"bound": "mfma", "achieved": f["useful_tflops"], "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(f["useful_tflops"] / 2500.0, 5), "traffic": None, "kernel": "k_gemm<1>"},
+            "modes": res,
+            "cpu_baseline": {"value": round(2 * len(rows) * N * N / dt / 1e9, 3), "unit": "GFLOP/s",
+                             "cores": min(os.cpu_count() or 1, 16), "kind": "port",
+                             "sample": f"{len(rows)} rows of C with the oracle's float64 blocked loop (no BLAS on the host)"}}
+
+
+def bench_laplacian(args):
+    """FP64 SpGEMM of the 2D 5-point Laplacian on a 4096 x 4096 grid: C = A * A."""
+    from bcl_amd import gen
+    from bcl_amd.csr import CSR
+    from bcl_amd.device import DeviceCSR, Handle, spgemm
+    O = _ref_or_oracle()
+    g = args.lap_grid
+    a = gen.laplacian2d(g)
+    h = Handle(0)
+    A = DeviceCSR.from_host(a)
+    C = spgemm(h, A, A)
+    ms, _ = time_steps(lambda: spgemm(h, A, A), args.steps, args.warmup, h.stream)
+    ms_step = ms / args.steps
+    products = int(np.diff(a.row_ptr)[a.col_idx].sum())
+    # exact check on a band of rows against the oracle (integer values: exact in fp64)
+    r0, r1 = g * (g // 2), g * (g // 2) + 4 * g
+    c_rp = C.row_ptr[r0:r1 + 1].cpu().numpy()
+    sub = CSR(r1 - r0, a.n, (a.row_ptr[r0:r1 + 1] - a.row_ptr[r0]).copy(), a.col_idx[a.row_ptr[r0]:a.row_ptr[r1]].copy(),
+              a.vals[a.row_ptr[r0]:a.row_ptr[r1]].copy())
+    ref = O.spgemm(sub, a)
+    got_col = C.col_idx[int(c_rp[0]):int(c_rp[-1])].cpu().numpy()
+    got_val = C.vals[int(c_rp[0]):int(c_rp[-1])].cpu().numpy()
+    assert np.array_equal(got_col, ref.col_idx) and np.array_equal(got_val, ref.vals)
+    alg = 2 * csr_bytes(a.m, a.nnz, 8, 8) + (a.m + 1) * 8 + C.nnz * 12
+    gbs = alg / (ms_step * 1e-3) / 1e9
+    t = time.perf_counter()
+    sub2 = CSR(g * 64, a.n, a.row_ptr[:g * 64 + 1].copy(), a.col_idx[:a.row_ptr[g * 64]].copy(),
+               a.vals[:a.row_ptr[g * 64]].copy())
+    (O.ref_spgemm if O.have_ref() else O.spgemm)(sub2, a)
+    dt = time.perf_counter() - t
+    p2 = int(np.diff(a.row_ptr)[sub2.col_idx].sum())
+    return {"metric": "FP64 SpGEMM GFLOP/s + HBM GB/s (2D 5-point Laplacian squared)",
+            "value": round(2 * products / (ms_step * 1e-3) / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"laplacian2d-{g}x{g}-squared", "nnz_a": a.nnz, "products": products,
+                       "nnz_c": C.nnz, "nnz_c_expected": 218021892 if g == 4096 else None},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "algorithmic_bytes": alg,
+                         "kernel": "whole two-phase call"},
+            "cpu_baseline": {"value": round(2 * p2 / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
+                             "kind": "reference" if O.have_ref() else "port",
+                             "sample": f"first {g * 64} rows, serial BCL::gemm(CSRMatrix<double>), {dt:.1f} s"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="stage", choices=["stage"])
+    ap.add_argument("--workload", default="stage", choices=["stage", "config1", "spmm", "gemm", "laplacian"])
+    ap.add_argument("--spmm-scale", type=int, default=20)
+    ap.add_argument("--gemm-n", type=int, default=8192)
+    ap.add_argument("--lap-grid", type=int, default=4096)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--grid", type=int, default=8)
     ap.add_argument("--stage-k", type=int, default=3)
@@ -226,7 +415,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only: the stage's products run on one MI355X; "
                                    "multi-GPU SUMMA (tile broadcasts over RCCL/xGMI) is a design, not built"}))
         sys.exit(2)
-    out = bench_stage(args)
+    fn = {"stage": bench_stage, "config1": bench_config1, "spmm": bench_spmm, "gemm": bench_gemm,
+          "laplacian": bench_laplacian}[args.workload]
+    out = fn(args)
     print(json.dumps(out), flush=True)
 
 
