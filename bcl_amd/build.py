@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-fvisibility=hidden", "-I" + os.path.join(ROOT, "include"), "-Wno-unused-result",
 ]
 HIP_SOURCES = ["core.hip", "spgemm.hip", "stage.hip", "mlp.hip", "spmm.hip", "gemm.hip", "accumulate.hip"]
-CPP_SOURCES = ["stubs.cpp"]
+CPP_SOURCES: list = []
 
 
 def _run(cmd):

@@ -83,6 +83,7 @@ void FlatRows<T, RP>::setup(bclx_handle* h_, const std::vector<TileDev<T, RP>>& 
   size_t o_bi = A.reserve(sizeof(int32_t) * kNumBins * nflat);
   size_t o_oc = A.reserve(sizeof(int32_t) * 2);
   size_t o_oi = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
+  size_t o_pi = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
   size_t o_cur = A.reserve(sizeof(unsigned long long) * 2);
   size_t o_stats = A.reserve(sizeof(unsigned long long) * 8);
   size_t nsb = (size_t)((nflat + kScanTile - 1) / kScanTile) + 1;
@@ -100,6 +101,8 @@ void FlatRows<T, RP>::setup(bclx_handle* h_, const std::vector<TileDev<T, RP>>& 
   R.bin_stride = nflat;
   R.ovf_count = A.at<int32_t>(o_oc);
   R.ovf_items = A.at<int32_t>(o_oi);
+  R.pred_count = R.ovf_count + 1;
+  R.pred_items = A.at<int32_t>(o_pi);
   R.cursor = A.at<unsigned long long>(o_cur);
   R.stats = A.at<unsigned long long>(o_stats);
   scan_partial = A.at<long long>(o_part);
@@ -125,8 +128,9 @@ void FlatRows<T, RP>::reset_counters(hipStream_t s) {
 }
 
 template <typename T, typename RP>
-void FlatRows<T, RP>::analyze(hipStream_t s, bool do_bin) {
-  hipLaunchKernelGGL((k_analyze<T, RP>), dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, R, do_bin ? 1 : 0);
+void FlatRows<T, RP>::analyze(hipStream_t s, bool do_bin, long long pred_min) {
+  hipLaunchKernelGGL((k_analyze<T, RP>), dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, R, do_bin ? 1 : 0,
+                     pred_min);
   BCLX_HIP(hipGetLastError());
 }
 

@@ -38,7 +38,8 @@ struct FlatRows {
   void setup(bclx_handle* h, const std::vector<TileDev<T, RP>>& tiles, bool with_cap);
   void upload_tiles();
   void reset_counters(hipStream_t s);
-  void analyze(hipStream_t s, bool do_bin);
+  // pred_min >= 0 (learned path): also write cap = min(prod, n) and list the rows with more than pred_min products
+  void analyze(hipStream_t s, bool do_bin, long long pred_min = -1);
   void bin(hipStream_t s, int src);
   // exclusive scan of cnt over all flat rows into off (nflat + 1 entries), plus off_add; total -> *scan_total
   void scan(hipStream_t s, const int32_t* cnt, int64_t* off, int64_t off_add);

@@ -81,6 +81,8 @@ struct RowsDev {
   int64_t bin_stride;
   int32_t* ovf_count;        // overflow list (learned path)
   int32_t* ovf_items;
+  int32_t* pred_count;       // rows whose capacity the predictor sets (learned path)
+  int32_t* pred_items;
   unsigned long long* cursor;  // bump allocator for rows placed outside their predicted slot
   unsigned long long* stats;   // [0] products [1] heavy rows [2] overflow rows [3] capacity exceeded [4] predicted
   long long out_limit;         // entries available in the output arrays; writes beyond it are refused
@@ -146,17 +148,21 @@ __device__ inline V block_sum(V v, V* smem) {
 // ------------------------------------------------------------------------------------------------------------
 
 template <typename T, typename RP>
-__global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_bin) {
+__global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_bin, long long pred_min) {
   __shared__ int s_long[kRowBlock];
   __shared__ int s_nlong;
   __shared__ int s_bcount[kNumBins], s_bbase[kNumBins];
+  __shared__ int s_npred, s_pbase;
   __shared__ long long s_red[kRowBlock / 64 + 1];
   const int tid = threadIdx.x;
   const int64_t fr = (int64_t)blockIdx.x * kRowBlock + tid;
   const TileDev<T, RP> tile = R.tiles[R.blk_tile[blockIdx.x]];
   const int64_t i = fr - tile.flat0;
   const bool valid = i >= 0 && i < tile.m;
-  if (tid == 0) s_nlong = 0;
+  if (tid == 0) {
+    s_nlong = 0;
+    s_npred = 0;
+  }
   if (tid < kNumBins) s_bcount[tid] = 0;
   __syncthreads();
   long long prod = 0;
@@ -189,6 +195,18 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_
   {
     const long long ub0 = prod < (long long)tile.n ? prod : (long long)tile.n;
     if (valid && bin_for(ub0, prod, R.tn) == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
+    if (pred_min >= 0) {
+      // learned path: every row starts at the exact bound min(prod, n); rows with more than pred_min products go
+      // on the list whose capacities the predictor then lowers
+      R.cap[fr] = valid ? (int32_t)ub0 : 0;
+      const bool pr = valid && prod > pred_min;
+      int ppos = 0;
+      if (pr) ppos = atomicAdd(&s_npred, 1);
+      __syncthreads();
+      if (tid == 0 && s_npred > 0) s_pbase = atomicAdd(R.pred_count, s_npred);
+      __syncthreads();
+      if (pr) R.pred_items[s_pbase + ppos] = (int32_t)fr;
+    }
   }
   if (do_bin) {
     const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;

@@ -9,6 +9,7 @@
 // Modes 1 and 2 are sized once at plan time by an eager run and then captured in a HIP graph (LDS bins run as
 // parallel graph branches), so bclx_stage_execute is one graph launch.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 
@@ -17,35 +18,34 @@
 
 namespace bclx {
 
-constexpr int kPredictMinProducts = 32;  // rows with at most this many products use the exact bound min(prod, n)
+constexpr int kPredictMinProducts = 32;  // rows with at most this many products keep the exact bound min(prod, n)
+constexpr int kPredictBlock = 256;
 
+// Predicted capacities for the rows the analysis pass listed (more than kPredictMinProducts products); every other
+// row keeps the exact bound the analysis wrote. The list is dense, so every lane of a wavefront runs the network.
 template <typename T, typename RP>
-__global__ __launch_bounds__(kRowBlock) void k_predict(RowsDev<T, RP> R, const float* params, float margin, int slack,
-                                                       int use_mlp) {
+__global__ __launch_bounds__(kPredictBlock) void k_predict(RowsDev<T, RP> R, const float* params, float margin,
+                                                           int slack) {
   __shared__ float P[kNParam];
-  if (use_mlp)
-    for (int q = threadIdx.x; q < kNParam; q += blockDim.x) P[q] = params[q];
+  for (int q = threadIdx.x; q < kNParam; q += blockDim.x) P[q] = params[q];
   __syncthreads();
-  const int64_t fr = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
-  const TileDev<T, RP> tile = R.tiles[R.blk_tile[blockIdx.x]];
-  const int64_t i = fr - tile.flat0;
-  const bool valid = i >= 0 && i < tile.m;
-  const long long prod = R.prod[fr];
-  const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
-  long long cap = valid ? ub : 0;
-  int predicted = 0;
-  if (valid && use_mlp && prod > kPredictMinProducts) {
+  const int cnt = *R.pred_count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&R.stats[4], (unsigned long long)cnt);
+  for (int idx = blockIdx.x * kPredictBlock + threadIdx.x; idx < cnt; idx += gridDim.x * kPredictBlock) {
+    const int fr = R.pred_items[idx];
+    const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
+    const int64_t i = fr - tile.flat0;
+    const long long prod = R.prod[fr];
+    const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
     float f[kFeat];
     row_features<RP>(tile.a_rp, tile.a_col, tile.b_rp, tile.b_col, i, tile.n, 0, tile.avg_a, tile.avg_b, prod, f);
     const float y = mlp_forward_row(P, f);
     const double pred = exp2((double)fminf(fmaxf(y, 0.f), 40.f)) - 1.0;
     const long long c = (long long)ceil(pred * (1.0 + (double)margin)) + slack;
-    cap = c < ub ? c : ub;
+    long long cap = c < ub ? c : ub;
     if (cap < 1) cap = 1;
-    predicted = 1;
+    R.cap[fr] = (int32_t)cap;
   }
-  R.cap[fr] = (int32_t)cap;
-  if (predicted) atomicAdd(&R.stats[4], 1ull);
 }
 
 // Overflow rows of the predicted pass -> LDS bins sized by the exact upper bound min(prod, n).
@@ -189,15 +189,25 @@ struct StagePlan : StageBase {
     join(s);
   }
 
+  // Analysis with capacities (exact bound everywhere), then the network on the listed rows when weights are given.
+  void predict(hipStream_t s) {
+    const bool use_mlp = opts.h_mlp_params != nullptr;
+    F.analyze(s, false, use_mlp ? (long long)kPredictMinProducts : LLONG_MAX);
+    if (use_mlp) {
+      const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(div_up(F.nflat, kPredictBlock),
+                                                                            (int64_t)4 * h->num_cus));
+      hipLaunchKernelGGL((k_predict<T, RP>), dim3(nb), dim3(kPredictBlock), 0, s, F.R, params.as<float>(),
+                         opts.margin, opts.slack);
+      BCLX_HIP(hipGetLastError());
+    }
+  }
+
   void enqueue(hipStream_t s, bool persistent) {
     F.reset_counters(s);
     if (mode == 1) {
       fill_i32(s, F.R.nnz, F.nflat, 0);
       fill_u64(s, F.R.cursor, 1, (unsigned long long)main_cap);
-      F.analyze(s, false);
-      hipLaunchKernelGGL((k_predict<T, RP>), dim3((unsigned)F.nblocks), dim3(kRowBlock), 0, s, F.R,
-                         params.as<float>(), opts.margin, opts.slack, opts.h_mlp_params ? 1 : 0);
-      BCLX_HIP(hipGetLastError());
+      predict(s);
       F.scan(s, F.R.cap, F.R.off, 0);
       F.bin(s, 1);
       run_bins(s, 2, counts1, persistent);
@@ -250,11 +260,8 @@ struct StagePlan : StageBase {
     F.reset_counters(s);
     if (mode == 1) {
       fill_i32(s, F.R.nnz, F.nflat, 0);
-      F.analyze(s, false);
+      predict(s);
       setup_heavy(s);
-      hipLaunchKernelGGL((k_predict<T, RP>), dim3((unsigned)F.nblocks), dim3(kRowBlock), 0, s, F.R,
-                         params.as<float>(), opts.margin, opts.slack, opts.h_mlp_params ? 1 : 0);
-      BCLX_HIP(hipGetLastError());
       F.scan(s, F.R.cap, F.R.off, 0);
       int64_t tot_cap = 0;
       BCLX_HIP(hipMemcpyAsync(&tot_cap, F.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -275,7 +282,7 @@ struct StagePlan : StageBase {
         F.R.out_limit = arena_cap;
         fill_u64(s, F.R.cursor, 1, (unsigned long long)main_cap);
         fill_u64(s, F.R.stats + 2, 2, 0);
-        fill_i32(s, F.R.ovf_count, 2, 0);
+        fill_i32(s, F.R.ovf_count, 1, 0);
         fill_i32(s, F.R.nnz, F.nflat, 0);
         F.bin(s, 1);
         read_counts(counts1);
@@ -418,7 +425,7 @@ struct StagePlan : StageBase {
       st[1] = tot;
       st[2] = (int64_t)v[2];
       st[3] = (int64_t)v[4];
-      st[4] = mode == 1 ? arena_cap : final_cap;
+      st[4] = mode == 1 ? main_cap : final_cap;  // learned: sum of the row capacities (the slot region)
       st[6] = (int64_t)v[1];
       BCLX_REQUIRE(v[3] == 0, BCLX_ERR_OVERFLOW,
                    "stage output exceeded the capacity sized at plan time (inputs changed since the plan?)");

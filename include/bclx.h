@@ -11,11 +11,10 @@
  *       as reached through CSRMatrix::dot                          bcl/containers/sequential/CSRMatrix.hpp:229-235
  *       (GPU counterpart: spgemm_cusparse                          bcl/containers/experimental/cuda/sequential/
  *                                                                  cusparse_util.cuh:269-514)
- *   bclx_csr_add
- *       the C += part of BCL::gemm (pre-insert of C, spgemm.hpp:47-54) and the pairwise sum_cusparse
- *       (cusparse_util.cuh:86-249)
  *   bclx_accumulate_tiles
  *       SparseHashAccumulator::get_matrix                          bcl/containers/sequential/SparseHashAccumulator.hpp:101-182
+ *       also the C += part of BCL::gemm (pre-insert of C, spgemm.hpp:47-54: C + A*B is the sum of two tiles)
+ *       and the pairwise sum_cusparse (cusparse_util.cuh:86-249)
  *   bclx_stage_*  (batched SUMMA stage)
  *       the per-stage local products of BCL::gemm(SPMatrix, SPMatrix, SPMatrix)  spgemm.hpp:181-242
  *   bclx_spmm

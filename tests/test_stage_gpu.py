@@ -116,3 +116,58 @@ def test_stage_replay_with_new_values(handle, stage13):
     torch.cuda.synchronize()
     check_stage(plan, pairs, scale=4.0)
     plan.close()
+
+
+def _median_ms(h, plan, reps=7):
+    plan.execute()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(h.stream)
+        plan.execute()
+        e.record(h.stream)
+        torch.cuda.synchronize()
+        ts.append(s.elapsed_time(e))
+    return float(np.median(ts))
+
+
+@pytest.mark.parametrize("kind", ["banded", "rmat"])
+def test_predictor_gain(handle, kind):
+    """What the trained network gains over the method without it, on the same stage.
+
+    Without the network every row's capacity is the exact bound min(products, n); the exact pipeline runs the
+    symbolic pass instead. All three must give the oracle's result. The trained network must size the slot region
+    well below the bound (its point: smaller tables and output arena) with few overflows; the times are reported.
+    """
+    params = predictor.load()
+    if params is None:
+        pytest.skip("no trained predictor weights in bcl_amd/data")
+    if kind == "banded":
+        a = gen.banded(1 << 15, 1 << 15, bw=64, max_per_row=24, seed=11)
+        grid, k = 2, 0
+    else:
+        a = gen.rmat(15, seed=12)
+        grid, k = 4, 1
+    tiles = gen.split_grid(a, grid, grid)
+    A, B, pairs = stage_tiles(tiles, tiles, grid, k)
+    res = {}
+    for name, mode, prm in (("trained", LEARNED, params), ("bound", LEARNED, None), ("exact", BATCHED_EXACT, None)):
+        print(f"[gain] {kind} {name}", flush=True)
+        plan = StagePlan(handle, A, B, mode=mode, params=prm)
+        ms = _median_ms(handle, plan)
+        st = plan.stats()
+        check_stage(plan, pairs)
+        res[name] = (ms, st)
+        plan.close()
+    t, b, x = res["trained"], res["bound"], res["exact"]
+    nnz = t[1]["nnz"]
+    assert b[1]["nnz"] == nnz and x[1]["nnz"] == nnz
+    print(f"[gain] {kind}: nnz={nnz} products={t[1]['products']} | trained {t[0]:.3f} ms, slots {t[1]['capacity']}, "
+          f"overflow {t[1]['overflow_rows']}/{t[1]['predicted_rows']} | bound {b[0]:.3f} ms, slots {b[1]['capacity']}"
+          f" | exact {x[0]:.3f} ms", flush=True)
+    assert t[1]["predicted_rows"] > 0
+    assert t[1]["overflow_rows"] <= 0.02 * t[1]["predicted_rows"] + 8
+    assert t[1]["capacity"] < b[1]["capacity"]
+    if kind == "banded":  # products / nnz is about 1.5 here; the trained capacities carry a 15% margin
+        assert t[1]["capacity"] < 0.9 * b[1]["capacity"]
