@@ -266,9 +266,14 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
 
 template <typename T, typename RP>
 void FlatRows<T, RP>::compact(hipStream_t s, const int64_t* dst_off, int* dst_col, T* dst_val, const int* src_col,
-                              const T* src_val) {
-  hipLaunchKernelGGL((k_compact<T, RP>), dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, R, dst_off, dst_col, dst_val,
-                     src_col, src_val);
+                              const T* src_val, int64_t max_out, int32_t* chunk_row) {
+  hipLaunchKernelGGL(k_chunk_rows, dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, dst_off, nflat, (long long)max_out,
+                     chunk_row);
+  BCLX_HIP(hipGetLastError());
+  const int64_t nchunks = std::max<int64_t>(1, div_up(max_out, kCompactChunk));
+  BCLX_REQUIRE(nchunks < INT32_MAX, BCLX_ERR_OVERFLOW, "output too large for one compaction launch");
+  hipLaunchKernelGGL((k_compact<T, RP>), dim3((unsigned)nchunks), dim3(kRowBlock), 0, s, R, dst_off, nflat, chunk_row,
+                     (long long)max_out, dst_col, dst_val, src_col, src_val);
   BCLX_HIP(hipGetLastError());
 }
 

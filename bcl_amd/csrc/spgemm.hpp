@@ -44,7 +44,9 @@ struct FlatRows {
   // exclusive scan of cnt over all flat rows into off (nflat + 1 entries), plus off_add; total -> *scan_total
   void scan(hipStream_t s, const int32_t* cnt, int64_t* off, int64_t off_add);
   void run_bin(hipStream_t s, int mode, int b, int64_t count_hint);
-  void compact(hipStream_t s, const int64_t* dst_off, int* dst_col, T* dst_val, const int* src_col, const T* src_val);
+  // max_out >= dst_off[nflat]; chunk_row holds div_up(max_out, kCompactChunk) entries
+  void compact(hipStream_t s, const int64_t* dst_off, int* dst_col, T* dst_val, const int* src_col, const T* src_val,
+               int64_t max_out, int32_t* chunk_row);
   // heavy rows: allocate the zeroed accumulators of the heavy kernel (before any graph capture)
   void prepare_heavy();
   int64_t read_stat(int q);

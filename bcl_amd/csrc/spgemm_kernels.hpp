@@ -680,7 +680,9 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
     __threadfence();
     __syncthreads();
     const long long base = s_base;
-    for (int w = w0; w < w1; w++) {
+    // words are dealt round-robin: a row's columns cluster (R-MAT hubs sit at low ids), so contiguous word ranges
+    // per thread would leave one thread emitting most of the row
+    for (int w = tid; w < W; w += kHRBlock) {
       unsigned word = s_bits[w];
       int r = s_pre[w];
       while (word) {
@@ -862,51 +864,80 @@ static __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int32_t*
   if (r0 + kScanPer >= n && r0 < n) off[n] = run;  // the thread holding the last element writes the total
 }
 
-// Copy each row's entries from the arena (at src_off[fr], nnz[fr] long) to the final CSR (at dst_off[fr]).
-// Block = 256 flat rows; threads stride over the block's contiguous output range (load balanced).
+// Compaction of the learned path: each row's entries move from the arena (at off[fr], nnz[fr] long) to the final
+// CSR (at dst_off[fr]). Work is split by output entries, not rows, so a block of hub rows (tens of thousands of
+// entries each) costs the same as any other: block c copies entries [c * kCompactChunk, (c + 1) * kCompactChunk).
+constexpr int kCompactChunk = 8192;
+
+// chunk_row[c] = the flat row holding output entry c * kCompactChunk.
+static __global__ __launch_bounds__(kRowBlock) void k_chunk_rows(const int64_t* dst_off, int64_t nflat,
+                                                                 long long max_out, int32_t* chunk_row) {
+  const int64_t r = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
+  if (r >= nflat) return;
+  const long long a = dst_off[r], b = dst_off[r + 1] < max_out ? dst_off[r + 1] : max_out;
+  for (long long c = (a + kCompactChunk - 1) / kCompactChunk; c * kCompactChunk < b; c++) chunk_row[c] = (int32_t)r;
+}
+
+// One block per chunk. The block walks windows of 256 row offsets (LDS) from the chunk's first row; each thread
+// finds the row of its entry by binary search in the window and copies it. Loads and stores are unit-stride across
+// the wavefront, four entries in flight per thread.
 template <typename T, typename RP>
-__global__ __launch_bounds__(kRowBlock) void k_compact(RowsDev<T, RP> R, const int64_t* dst_off, int* dst_col,
+__global__ __launch_bounds__(kRowBlock) void k_compact(RowsDev<T, RP> R, const int64_t* dst_off, int64_t nflat,
+                                                       const int32_t* chunk_row, long long max_out, int* dst_col,
                                                        T* dst_val, const int* src_col, const T* src_val) {
-  // Copy each row's entries from the arena (at off[fr], nnz[fr] long) to the final CSR (at dst_off[fr]). A block
-  // takes 256 flat rows; its output range is contiguous, and each thread copies 4 consecutive outputs per step (one
-  // binary search for the first, then it walks row boundaries), so loads and stores stay coalesced and independent.
   __shared__ long long s_dst[kRowBlock + 1];
   __shared__ long long s_src[kRowBlock];
   const int tid = threadIdx.x;
-  const int64_t fr0 = (int64_t)blockIdx.x * kRowBlock;
-  s_dst[tid] = dst_off[fr0 + tid];
-  if (tid == 0) s_dst[kRowBlock] = dst_off[fr0 + kRowBlock];
-  s_src[tid] = R.off[fr0 + tid];
-  __syncthreads();
-  const long long lo = s_dst[0], hi = s_dst[kRowBlock];
-  for (long long e0 = lo + 4LL * tid; e0 < hi; e0 += 4LL * kRowBlock) {
-    int a = 0, b = kRowBlock;  // row r with s_dst[r] <= e0 < s_dst[r + 1]
-    while (b - a > 1) {
-      const int mid = (a + b) >> 1;
-      if (s_dst[mid] <= e0) a = mid; else b = mid;
-    }
-    int cv[4];
-    T vv[4];
-    long long src[4];
-    int r = a;
+  long long total = dst_off[nflat];
+  if (total > max_out) {  // more entries than the output holds: refuse the excess and flag it
+    if (blockIdx.x == 0 && tid == 0) R.stats[3] = 1ull;
+    total = max_out;
+  }
+  const long long c0 = (long long)blockIdx.x * kCompactChunk;
+  if (c0 >= total) return;
+  const long long c1 = c0 + kCompactChunk < total ? c0 + kCompactChunk : total;
+  int64_t r0 = chunk_row[blockIdx.x];
+  long long e = c0;
+  while (e < c1) {
+    const int64_t r = r0 + tid;
+    s_dst[tid] = dst_off[r < nflat ? r : nflat];
+    s_src[tid] = r < nflat ? R.off[r] : 0;
+    if (tid == 0) s_dst[kRowBlock] = dst_off[r0 + kRowBlock < nflat ? r0 + kRowBlock : nflat];
+    __syncthreads();
+    const long long we = s_dst[kRowBlock] < c1 ? s_dst[kRowBlock] : c1;
+    for (long long x0 = e; x0 < we; x0 += 4LL * kRowBlock) {
+      long long src[4];
+      bool ok[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const long long e = e0 + q;
-      while (r < kRowBlock - 1 && s_dst[r + 1] <= e) r++;
-      src[q] = s_src[r] + (e - s_dst[r]);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-      if (e0 + q < hi) {
-        cv[q] = src_col[src[q]];
-        vv[q] = src_val[src[q]];
+      for (int q = 0; q < 4; q++) {
+        const long long x = x0 + (long long)q * kRowBlock + tid;
+        ok[q] = x < we;
+        int a = 0, b = kRowBlock;  // largest a with s_dst[a] <= x
+        while (b - a > 1) {
+          const int mid = (a + b) >> 1;
+          if (s_dst[mid] <= x) a = mid; else b = mid;
+        }
+        src[q] = s_src[a] + (x - s_dst[a]);
       }
+      int cv[4];
+      T vv[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++)
-      if (e0 + q < hi) {
-        dst_col[e0 + q] = cv[q];
-        dst_val[e0 + q] = vv[q];
-      }
+      for (int q = 0; q < 4; q++)
+        if (ok[q]) {
+          cv[q] = src_col[src[q]];
+          vv[q] = src_val[src[q]];
+        }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (ok[q]) {
+          const long long x = x0 + (long long)q * kRowBlock + tid;
+          dst_col[x] = cv[q];
+          dst_val[x] = vv[q];
+        }
+    }
+    e = we;
+    r0 += kRowBlock;
+    __syncthreads();
   }
 }
 

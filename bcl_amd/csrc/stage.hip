@@ -96,7 +96,7 @@ struct StagePlan : StageBase {
   std::vector<int64_t> c_nnz;
   // modes 1 / 2
   FlatRows<T, RP> F;
-  DevBuf params, arena_col, arena_val, final_col, final_val, final_off, tile_rp;
+  DevBuf params, arena_col, arena_val, final_col, final_val, final_off, tile_rp, chunk_row;
   int64_t arena_cap = 0, final_cap = 0, main_cap = 0;
   int32_t counts1[kNumBins] = {0}, counts2[kNumBins] = {0};
   std::vector<hipStream_t> side;
@@ -218,7 +218,7 @@ struct StagePlan : StageBase {
       run_bins(s, 3, counts2, persistent);
       F.scan(s, F.R.nnz, final_off.as<int64_t>(), 0);
       F.compact(s, final_off.as<int64_t>(), final_col.as<int>(), final_val.as<T>(), arena_col.as<int>(),
-                arena_val.as<T>());
+                arena_val.as<T>(), final_cap, chunk_row.as<int32_t>());
     } else {
       F.analyze(s, true);
       run_bins(s, 0, counts1, persistent);
@@ -312,6 +312,7 @@ struct StagePlan : StageBase {
       final_cap = tot + tot / 64 + 4096;
       final_col.alloc(sizeof(int) * final_cap);
       final_val.alloc(sizeof(T) * final_cap);
+      chunk_row.alloc(sizeof(int32_t) * (div_up(final_cap, kCompactChunk) + 1));
     } else {
       F.analyze(s, true);
       setup_heavy(s);
