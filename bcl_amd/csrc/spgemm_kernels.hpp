@@ -671,33 +671,34 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
       atomicAdd(&R.stats[1], 1ull);
     }
     __syncthreads();
+    // The scratch is private to this workgroup, so workgroup-scope atomics (performed in L2) and the workgroup
+    // barrier order every add before the read-and-reset below; no device-scope fence (an L2 write-back) is needed.
     for_each_product<kHRBlock, kHRBlock, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
                                                        [&](int j, double v) {
                                                          const int w = j >> 5;
                                                          const int r = s_pre[w] + __popc(s_bits[w] & ((1u << (j & 31)) - 1u));
-                                                         unsafeAtomicAdd(&acc[r], v);
+                                                         __hip_atomic_fetch_add(&acc[r], v, __ATOMIC_RELAXED,
+                                                                                __HIP_MEMORY_SCOPE_WORKGROUP);
                                                        });
-    __threadfence();
     __syncthreads();
     const long long base = s_base;
-    // words are dealt round-robin: a row's columns cluster (R-MAT hubs sit at low ids), so contiguous word ranges
-    // per thread would leave one thread emitting most of the row
-    for (int w = tid; w < W; w += kHRBlock) {
-      unsigned word = s_bits[w];
-      int r = s_pre[w];
-      while (word) {
-        const int bit = __ffs(word) - 1;
-        const double v = __hip_atomic_load(&acc[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (base >= 0) {
-          tile.c_col[base + r] = w * 32 + bit;
-          tile.c_val[base + r] = (T)v;
-        }
-        acc[r] = 0.0;
-        r++;
-        word &= word - 1;
-      }
+    // values: the scratch is already in output (rank) order, so this pass is unit-stride; it also re-zeroes the
+    // scratch for the next row, whether or not this row was written
+    for (long long r = tid; r < total; r += kHRBlock) {
+      const double v = __hip_atomic_exchange(&acc[r], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (base >= 0) tile.c_val[base + r] = (T)v;
     }
-    __threadfence();
+    // columns: words dealt round-robin (a row's columns cluster at the R-MAT hubs' low ids, so contiguous word
+    // ranges per thread would leave one thread writing most of the row)
+    if (base >= 0)
+      for (int w = tid; w < W; w += kHRBlock) {
+        unsigned word = s_bits[w];
+        long long r = base + s_pre[w];
+        while (word) {
+          tile.c_col[r++] = w * 32 + (__ffs(word) - 1);
+          word &= word - 1;
+        }
+      }
     __syncthreads();
   }
 }
@@ -709,6 +710,19 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
 //   MODE 0 symbolic -> nnz[fr]; MODE 1 numeric at off[fr];
 //   MODE 2 predicted: write at off[fr] when count <= cap[fr], else bump-allocate; MODE 3 bump-allocate.
 // ------------------------------------------------------------------------------------------------------------
+
+// The dense accumulator (and the bitmap when it is in global memory) is private to one workgroup: every access is
+// a workgroup-scope atomic (performed in L2) ordered by workgroup barriers, so no device-scope fence is needed.
+template <bool LDS_BITMAP>
+__device__ inline void heavy_bit_set(unsigned* bits, int j) {
+  if (LDS_BITMAP) atomicOr(&bits[j >> 5], 1u << (j & 31));
+  else __hip_atomic_fetch_or(&bits[j >> 5], 1u << (j & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <bool LDS_BITMAP>
+__device__ inline unsigned heavy_bit_word(unsigned* bits, int64_t w) {
+  if (LDS_BITMAP) return bits[w];
+  return __hip_atomic_load(&bits[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 template <int MODE, bool LDS_BITMAP, typename T, typename RP>
 __global__ __launch_bounds__(kHeavyThreads) void k_heavy(RowsDev<T, RP> R, int bin, double* dense_slots,
@@ -744,23 +758,22 @@ __global__ __launch_bounds__(kHeavyThreads) void k_heavy(RowsDev<T, RP> R, int b
       const RP b0 = tile.b_rp[kk], b1 = tile.b_rp[kk + 1];
       for (RP q = b0 + sl; q < b1; q += L) {
         const int j = tile.b_col[q];
-        atomicOr(&bits[j >> 5], 1u << (j & 31));
-        if (NUM) unsafeAtomicAdd(&dense[j], av * (double)tile.b_val[q]);
+        heavy_bit_set<LDS_BITMAP>(bits, j);
+        if (NUM)
+          __hip_atomic_fetch_add(&dense[j], av * (double)tile.b_val[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
-    __threadfence_block();
-    if (!LDS_BITMAP || NUM) __threadfence();
     __syncthreads();
     const int64_t per = (W + kHeavyThreads - 1) / kHeavyThreads;
     const int64_t w0 = tid * per, w1 = (w0 + per < W) ? w0 + per : W;
     long long c = 0;
-    for (int64_t w = w0; w < w1; w++) c += __popc(bits[w]);
+    for (int64_t w = w0; w < w1; w++) c += __popc(heavy_bit_word<LDS_BITMAP>(bits, w));
     long long total;
     const long long pre = block_excl_scan<kHeavyThreads, long long>(c, &total, s_scan);
     if (MODE == 0) {
       if (tid == 0) R.nnz[fr] = (int)total;
       if (!LDS_BITMAP)
-        for (int64_t w = w0; w < w1; w++) bits[w] = 0u;
+        for (int64_t w = w0; w < w1; w++) __hip_atomic_store(&bits[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     } else {
       if (tid == 0) {
         long long b;
@@ -783,26 +796,22 @@ __global__ __launch_bounds__(kHeavyThreads) void k_heavy(RowsDev<T, RP> R, int b
       const bool fits = s_base >= 0;
       long long pos = s_base + pre;
       for (int64_t w = w0; w < w1; w++) {
-        unsigned word = bits[w];
-        while (word && fits) {
+        unsigned word = heavy_bit_word<LDS_BITMAP>(bits, w);
+        while (word) {  // read and re-zero the accumulator whether or not the row is written
           const int bit = __ffs(word) - 1;
           const int j = (int)(w * 32 + bit);
-          tile.c_col[pos] = j;
-          tile.c_val[pos] = (T)dense[j];
-          pos++;
+          const double v = __hip_atomic_exchange(&dense[j], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (fits) {
+            tile.c_col[pos] = j;
+            tile.c_val[pos] = (T)v;
+            pos++;
+          }
           word &= word - 1;
         }
-        word = bits[w];
-        while (word) {  // re-zero the accumulator whether or not the row was written
-          const int bit = __ffs(word) - 1;
-          dense[w * 32 + bit] = 0.0;
-          word &= word - 1;
-        }
-        if (!LDS_BITMAP) bits[w] = 0u;
+        if (!LDS_BITMAP) __hip_atomic_store(&bits[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
     if (tid == 0) atomicAdd(&R.stats[1], 1ull);
-    __threadfence();
     __syncthreads();
   }
 }

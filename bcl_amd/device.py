@@ -152,7 +152,8 @@ def gemm(h: Handle, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, mode: int
     """C = beta * C + alpha * op(A) @ op(B) in place, row-major fp32 (cblas_gemm_wrapper_ with RowMajor).
     mode: GEMM_FP32 (FP32 MFMA), GEMM_BF16 (bf16 inputs, fp32 accumulation), GEMM_SPLIT_BF16 (3-term split)."""
     for t in (A, B, C):
-        assert t.dim() == 2 and (t.stride(1) == 1 or t.shape[1] <= 1) and t.dtype == torch.float32
+        assert t.dim() == 2 and t.dtype == torch.float32
+        assert t.numel() == 0 or t.stride(1) == 1 or t.shape[1] <= 1, "rows must be contiguous"
     m, k = (A.shape[1], A.shape[0]) if trans_a else (A.shape[0], A.shape[1])
     kb, n = (B.shape[1], B.shape[0]) if trans_b else (B.shape[0], B.shape[1])
     assert k == kb and C.shape[0] >= m and C.shape[1] >= n
