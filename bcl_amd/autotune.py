@@ -22,15 +22,17 @@ import time
 import numpy as np
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
-DEFAULT = {"work_per_lane": 64, "heavy_need": 4096, "load_shift": 1, "margin": 0.15, "slack": 4}
-SPACE = {"work_per_lane": [16, 32, 64, 128, 256], "heavy_need": [1024, 2048, 4096], "load_shift": [1, 2],
-         "margin": [0.05, 0.1, 0.15, 0.25, 0.4]}
+DEFAULT = {"work_per_lane": 64, "heavy_need": 4096, "load_shift": 1, "esc_min": 32, "esc_max": 4096,
+           "margin": 0.15, "slack": 4}
+SPACE = {"esc_max": [0, 512, 1024, 2048, 4096], "esc_min": [0, 16, 32, 64], "work_per_lane": [16, 32, 64, 128, 256],
+         "heavy_need": [1024, 2048, 4096], "load_shift": [1, 2], "margin": [0.05, 0.1, 0.15, 0.25, 0.4]}
+KNOBS = ("work_per_lane", "heavy_need", "load_shift", "esc_min", "esc_max")
 
 
 def time_plan(h, A, B, params, cfg, reps=3):
     import torch
     from .stage import LEARNED, StagePlan
-    tun = np.array([cfg["work_per_lane"], cfg["heavy_need"], cfg["load_shift"]], np.int32)
+    tun = np.array([cfg[k] for k in KNOBS], np.int32)
     plan = StagePlan(h, A, B, mode=LEARNED, params=params, margin=cfg["margin"], slack=cfg["slack"], tuning=tun)
     plan.execute()
     torch.cuda.synchronize()
@@ -87,7 +89,7 @@ def main():
     params = predictor.load()
     cfg, best, trials = tune(h, A, B, params, log=lambda s: print(s, flush=True))
     os.makedirs(a.out, exist_ok=True)
-    out = {"tuning": [cfg["work_per_lane"], cfg["heavy_need"], cfg["load_shift"]], "margin": cfg["margin"],
+    out = {"tuning": [cfg[k] for k in KNOBS], "margin": cfg["margin"],
            "slack": cfg["slack"], "ms": best, "default_ms": trials[0][1],
            "workload": f"rmat-s{a.scale}-ef16-grid{a.grid}-stage{a.stage_k}",
            "trials": [{"cfg": c, "ms": m} for c, m in trials], "seconds": time.time() - t0}

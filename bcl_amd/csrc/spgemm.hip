@@ -198,10 +198,54 @@ int rows_per_block(int b) {
   return BLOCK / G;
 }
 
+template <int K, typename T, typename RP>
+void launch_esc(const RowsDev<T, RP>& R, hipStream_t s, int mode, int b, unsigned grid) {
+  switch (mode) {
+    case 0: hipLaunchKernelGGL((k_esc<K, 0, T, RP>), dim3(grid), dim3(64 * kEscWaves), 0, s, R, b); break;
+    case 1: hipLaunchKernelGGL((k_esc<K, 1, T, RP>), dim3(grid), dim3(64 * kEscWaves), 0, s, R, b); break;
+    case 2: hipLaunchKernelGGL((k_esc<K, 2, T, RP>), dim3(grid), dim3(64 * kEscWaves), 0, s, R, b); break;
+    default: hipLaunchKernelGGL((k_esc<K, 3, T, RP>), dim3(grid), dim3(64 * kEscWaves), 0, s, R, b); break;
+  }
+  BCLX_HIP(hipGetLastError());
+}
+
+template <int W, typename T, typename RP>
+void launch_esc_block(const RowsDev<T, RP>& R, hipStream_t s, int mode, int b, unsigned grid) {
+  switch (mode) {
+    case 0: hipLaunchKernelGGL((k_esc_block<W, 0, T, RP>), dim3(grid), dim3(64 * W), 0, s, R, b); break;
+    case 1: hipLaunchKernelGGL((k_esc_block<W, 1, T, RP>), dim3(grid), dim3(64 * W), 0, s, R, b); break;
+    case 2: hipLaunchKernelGGL((k_esc_block<W, 2, T, RP>), dim3(grid), dim3(64 * W), 0, s, R, b); break;
+    default: hipLaunchKernelGGL((k_esc_block<W, 3, T, RP>), dim3(grid), dim3(64 * W), 0, s, R, b); break;
+  }
+  BCLX_HIP(hipGetLastError());
+}
+
 template <typename T, typename RP>
 void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint) {
   // count_hint >= 0: exact row count known on the host (grid sized to it); < 0: persistent grid.
   if (b == 0 || count_hint == 0) return;
+  if (b >= kEscBin0 && b < kEscBin0 + kEscWaveBins) {
+    unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, div_up(count_hint, kEscWaves))
+                                    : (unsigned)(h->num_cus * 8);
+    grid = std::min<unsigned>(grid, 1u << 20);
+    switch (b - kEscBin0) {
+      case 0: launch_esc<1>(R, s, mode, b, grid); break;
+      case 1: launch_esc<2>(R, s, mode, b, grid); break;
+      case 2: launch_esc<4>(R, s, mode, b, grid); break;
+      default: launch_esc<8>(R, s, mode, b, grid); break;
+    }
+    return;
+  }
+  if (b >= kEscBin0 + kEscWaveBins) {
+    unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, count_hint) : (unsigned)(h->num_cus * 8);
+    grid = std::min<unsigned>(grid, 1u << 20);
+    switch (b - kEscBin0 - kEscWaveBins) {
+      case 0: launch_esc_block<2>(R, s, mode, b, grid); break;
+      case 1: launch_esc_block<4>(R, s, mode, b, grid); break;
+      default: launch_esc_block<8>(R, s, mode, b, grid); break;
+    }
+    return;
+  }
   if (b == kHeavyBin && (max_n + 31) / 32 <= kHRWords) {
     const int nslots = (int)std::max<int64_t>(1, std::min<int64_t>(count_hint >= 0 ? count_hint : h->num_cus,
                                                                      (int64_t)h->num_cus));

@@ -12,8 +12,15 @@
 namespace bclx {
 
 constexpr int kEmptyKey = 0x7fffffff;  // larger than any column index
-constexpr int kNumBins = 11;           // 0: empty row; 1..9: LDS tables; 10: heavy (global accumulator)
+// 0: empty row; 1..9: LDS hash tables; 10: heavy (global accumulator); 11..17: sort-based (ESC) rows of up to
+// 64 * 2^(b - 11) products (bins 11..14: one wavefront per row; 15..17: 2, 4 or 8 wavefronts per row)
+constexpr int kNumBins = 18;
 constexpr int kHeavyBin = 10;
+constexpr int kEscBin0 = 11;
+constexpr int kEscBins = 7;
+constexpr int kEscWaveBins = 4;                         // bins 11..14 sort within one wavefront
+constexpr int kEscMaxProducts = 64 << (kEscBins - 1);  // 4096
+constexpr int kEscColBits = 19;                         // sort keys pack column (< 2^19) and product index (13 bits)
 constexpr int kRowBlock = 256;         // flat-row padding and analysis block size
 constexpr int kHeavyThreads = 1024;
 
@@ -29,10 +36,21 @@ struct BinTuning {
   int32_t work_per_lane = 64;     // a row's group must have >= products / work_per_lane lanes
   int32_t heavy_need = 4096;      // rows needing more table entries than this use the global accumulator
   int32_t load_shift = 1;         // table slots >= need << load_shift (1: load <= 1/2)
+  int32_t esc_min = 32;           // rows with esc_min < products <= esc_max sort instead of hashing (when the
+  int32_t esc_max = 4096;         //   tile is at most 2^19 columns wide); esc_max 0 turns sorting off
 };
 
-__host__ __device__ inline int bin_for(int64_t need, int64_t work, const BinTuning& tn) {
+// ESC bin of a row with `work` products: the smallest 64 * 2^e >= work.
+__host__ __device__ inline int esc_bin(int64_t work) {
+  int e = 0;
+  while ((64LL << e) < work) e++;
+  return kEscBin0 + e;
+}
+
+// esc_ok: the tile's columns fit the packed sort key (n <= 2^19).
+__host__ __device__ inline int bin_for(int64_t need, int64_t work, const BinTuning& tn, bool esc_ok = false) {
   if (work <= 0) return 0;
+  if (esc_ok && work > tn.esc_min && work <= tn.esc_max && work <= kEscMaxProducts) return esc_bin(work);
   if (need > tn.heavy_need) return kHeavyBin;
   int64_t slots = need << tn.load_shift;
   int bn = 1;
@@ -194,12 +212,14 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_
   R.prod[fr] = valid ? prod : 0;
   {
     const long long ub0 = prod < (long long)tile.n ? prod : (long long)tile.n;
-    if (valid && bin_for(ub0, prod, R.tn) == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
+    const int b0 = valid ? bin_for(ub0, prod, R.tn, tile.n <= (1 << kEscColBits)) : 0;
+    if (b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
     if (pred_min >= 0) {
-      // learned path: every row starts at the exact bound min(prod, n); rows with more than pred_min products go
-      // on the list whose capacities the predictor then lowers
+      // learned path: every row starts at the exact bound min(prod, n); rows with more than pred_min products that
+      // hash (sorted rows are exact in one pass and keep the bound) go on the list whose capacities the predictor
+      // then lowers
       R.cap[fr] = valid ? (int32_t)ub0 : 0;
-      const bool pr = valid && prod > pred_min;
+      const bool pr = valid && prod > pred_min && b0 < kEscBin0;
       int ppos = 0;
       if (pr) ppos = atomicAdd(&s_npred, 1);
       __syncthreads();
@@ -210,7 +230,7 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_
   }
   if (do_bin) {
     const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
-    const int b = valid ? bin_for(ub, prod, R.tn) : 0;
+    const int b = valid ? bin_for(ub, prod, R.tn, tile.n <= (1 << kEscColBits)) : 0;
     int pos = 0;
     if (b > 0) pos = atomicAdd(&s_bcount[b], 1);
     __syncthreads();
@@ -238,7 +258,7 @@ __global__ __launch_bounds__(kRowBlock) void k_bin(RowsDev<T, RP> R, int src) {
   if (valid) {
     const long long work = R.prod[fr];
     const long long need = src == 0 ? (long long)R.nnz[fr] : (long long)R.cap[fr];
-    b = (need == 0 && src == 0) ? 0 : bin_for(need, work, R.tn);
+    b = (need == 0 && src == 0) ? 0 : bin_for(need, work, R.tn, tile.n <= (1 << kEscColBits));
   }
   int pos = 0;
   if (b > 0) pos = atomicAdd(&s_bcount[b], 1);
@@ -587,6 +607,364 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
             tile.c_val[o + s] = (T)vals[s];
           }
         }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Sort-based rows (ESC: expand, sort, compress), one wavefront per row, for rows of 64 * K products or fewer.
+// Rows of a SUMMA-stage tile product are mostly output-dense (nnz(C_i) close to the products), where a hash table
+// costs two LDS slots per output entry plus a rank structure, and caps how many rows a CU holds. Here the products
+// of a row are expanded into registers as 32-bit keys (column << 13 | product index), sorted with a bitonic network
+// across the wavefront (registers and lane shuffles), and equal columns are summed in product order (the reference's
+// A-row order, in float64, rounded once). LDS holds only the float64 products (8 bytes per product) and the sorted
+// columns. The count is exact after the sort, so the predicted mode never overflows: a row that outgrows its slot
+// is bump-allocated in the same pass.
+//   MODE 0 symbolic -> nnz[fr]; MODE 1 numeric at off[fr]; MODE 2 at off[fr] when nnz <= cap[fr], else
+//   bump-allocated; MODE 3 bump-allocated.
+// ------------------------------------------------------------------------------------------------------------
+
+constexpr int kEscWaves = 4;  // wavefronts per block, each on its own row
+
+constexpr unsigned kEscKeyShift = 32 - kEscColBits;      // key = column << 13 | product index
+constexpr unsigned kEscIdxMask = (1u << kEscKeyShift) - 1u;
+constexpr unsigned kEscPad = 0xffffffffu;                 // sorts after every real key
+
+// Does sorted position k * 64 + lane start an output entry (a real key whose column differs from its
+// predecessor's)? Position 0 always does: the pad key's column field equals the largest column.
+template <int K>
+__device__ inline bool esc_head(const unsigned (&key)[K], int k, int lane) {
+  const unsigned up = (unsigned)__shfl_up((int)key[k], 1, 64);
+  const unsigned wrap = k > 0 ? (unsigned)__shfl((int)key[k > 0 ? k - 1 : 0], 63, 64) : kEscPad;
+  const unsigned prev = lane == 0 ? wrap : up;
+  return key[k] != kEscPad && ((k == 0 && lane == 0) || (prev >> kEscKeyShift) != (key[k] >> kEscKeyShift));
+}
+
+// Compare-exchange of one bitonic stage whose partner is `d` lanes away (d < 64): keep min or max of the pair.
+template <int D>
+__device__ inline unsigned esc_xchg(unsigned v, bool keep_min) {
+  const unsigned o = (unsigned)__shfl_xor((int)v, D, 64);
+  return keep_min ? min(v, o) : max(v, o);
+}
+
+// Bitonic sort, ascending, of the 64 * K keys of a wavefront; element e = k * 64 + lane lives in key[k].
+template <int K>
+__device__ inline void esc_sort(unsigned (&key)[K], int lane) {
+#pragma unroll
+  for (int size = 2; size <= 64 * K; size <<= 1) {
+#pragma unroll
+    for (int d = size >> 1; d > 0; d >>= 1) {
+      if (d >= 64) {
+        const int dk = d >> 6;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          if ((k & dk) == 0) {
+            const bool asc = ((k * 64) & size) == 0;
+            const unsigned a = key[k], b = key[k | dk];
+            const bool sw = (a > b) == asc;
+            key[k] = sw ? b : a;
+            key[k | dk] = sw ? a : b;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const bool asc = (((k * 64) | lane) & size) == 0;
+          const bool lower = (lane & d) == 0;
+          const bool keep_min = asc == lower;
+          switch (d) {
+            case 1: key[k] = esc_xchg<1>(key[k], keep_min); break;
+            case 2: key[k] = esc_xchg<2>(key[k], keep_min); break;
+            case 4: key[k] = esc_xchg<4>(key[k], keep_min); break;
+            case 8: key[k] = esc_xchg<8>(key[k], keep_min); break;
+            case 16: key[k] = esc_xchg<16>(key[k], keep_min); break;
+            default: key[k] = esc_xchg<32>(key[k], keep_min); break;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int K, int MODE, typename T, typename RP>
+__global__ __launch_bounds__(64 * kEscWaves) void k_esc(RowsDev<T, RP> R, int bin) {
+  constexpr int N = 64 * K;
+  constexpr bool NUM = MODE != 0;
+  __shared__ double s_val[kEscWaves][NUM ? N : 1];  // float64 products by product index
+  __shared__ unsigned s_key[kEscWaves][NUM ? N : 1];  // sorted keys
+  __shared__ int s_pre[kEscWaves][65];              // per chunk of A entries: products before each entry
+  __shared__ RP s_st[kEscWaves][64];                // B-row start of each entry
+  __shared__ T s_av[kEscWaves][64];                 // A value of each entry
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int count = R.bin_count[bin];
+  const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
+  for (int idx = blockIdx.x * kEscWaves + w; idx < count; idx += gridDim.x * kEscWaves) {
+    const int fr = items[idx];
+    const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
+    const int64_t i = fr - tile.flat0;
+    const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
+    unsigned key[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) key[k] = kEscPad;
+    // expand: product x (A-row order) -> key (column << 13 | x), float64 product -> s_val[x]
+    int base = 0;
+    for (RP c0 = a0; c0 < a1; c0 += 64) {
+      const int ne = (int)((a1 - c0) < (RP)64 ? (a1 - c0) : (RP)64);
+      int len = 0;
+      if (lane < ne) {
+        const int kk = tile.a_col[c0 + lane];
+        const RP st = tile.b_rp[kk];
+        len = (int)(tile.b_rp[kk + 1] - st);
+        s_st[w][lane] = st;
+        if (NUM) s_av[w][lane] = tile.a_val[c0 + lane];
+      }
+      int inc = len;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+      }
+      const int total = __shfl(inc, 63, 64);
+      if (lane < ne) s_pre[w][lane] = inc - len;
+      if (lane == 0) s_pre[w][ne] = total;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int x = k * 64 + lane;
+        const int t = x - base;
+        if (t >= 0 && t < total) {
+          int lo = 0, hi = ne;  // entry e with s_pre[e] <= t < s_pre[e + 1]
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_pre[w][mid] <= t) lo = mid; else hi = mid;
+          }
+          const RP q = s_st[w][lo] + (RP)(t - s_pre[w][lo]);
+          key[k] = ((unsigned)tile.b_col[q] << kEscKeyShift) | (unsigned)x;
+          if (NUM) s_val[w][x] = (double)s_av[w][lo] * (double)tile.b_val[q];
+        }
+      }
+      base += total;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    esc_sort<K>(key, lane);
+    // compress: a sorted position starts an output entry when its column differs from the previous position's
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) cnt += __popcll(__ballot(esc_head<K>(key, k, lane)));
+    if (MODE == 0) {
+      if (lane == 0) R.nnz[fr] = cnt;
+      continue;
+    }
+    long long o = 0;
+    if (lane == 0) {
+      if (MODE == 1 || (MODE == 2 && cnt <= R.cap[fr])) {
+        o = R.off[fr];
+      } else {
+        o = (long long)atomicAdd(R.cursor, (unsigned long long)cnt);
+        R.off[fr] = o;
+      }
+      if (o + cnt > R.out_limit) {
+        R.stats[3] = 1ull;
+        o = -1;
+      }
+      R.nnz[fr] = cnt;
+    }
+    o = __shfl(o, 0, 64);
+#pragma unroll
+    for (int k = 0; k < K; k++) s_key[w][k * 64 + lane] = key[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (o >= 0) {
+      int before = 0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const bool head = esc_head<K>(key, k, lane);
+        const unsigned long long mask = __ballot(head);
+        if (head) {
+          // sum the run of equal columns in sorted order, i.e. in product (A-row) order
+          const unsigned col = key[k] >> kEscKeyShift;
+          double sum = s_val[w][key[k] & kEscIdxMask];
+          for (int p = k * 64 + lane + 1; p < N; p++) {
+            const unsigned kp = s_key[w][p];
+            if (kp == kEscPad || (kp >> kEscKeyShift) != col) break;
+            sum += s_val[w][kp & kEscIdxMask];
+          }
+          const long long pos = o + before + __popcll(mask & ((1ull << lane) - 1ull));
+          tile.c_col[pos] = (int)col;
+          tile.c_val[pos] = (T)sum;
+        }
+        before += __popcll(mask);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// Sort-based rows of 512 * W products or fewer, W wavefronts per row: each wavefront expands and sorts 512 of the
+// row's products in registers (as the single-wavefront kernel does), the sorted runs are merged pairwise in LDS (each
+// key's merged position is its own index plus its rank in the partner run, found by binary search; ties go to the
+// left run, so the pad keys fill the tail), and the block compresses the sorted row. LDS: 16 bytes per product.
+constexpr int kEscChunk = 512;
+
+template <int W, int MODE, typename T, typename RP>
+__global__ __launch_bounds__(64 * W) void k_esc_block(RowsDev<T, RP> R, int bin) {
+  constexpr int N = kEscChunk * W;
+  constexpr int K = kEscChunk / 64;
+  constexpr bool NUM = MODE != 0;
+  __shared__ unsigned s_key[2][N];
+  __shared__ double s_val[NUM ? N : 1];
+  __shared__ int s_pre[W][65];
+  __shared__ RP s_st[W][64];
+  __shared__ T s_av[W][64];
+  __shared__ int s_cnt[W];
+  __shared__ long long s_o;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int count = R.bin_count[bin];
+  const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
+  for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
+    const int fr = items[idx];
+    const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
+    const int64_t i = fr - tile.flat0;
+    const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
+    unsigned key[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) key[k] = kEscPad;
+    // expand this wavefront's products x in [w * 512, (w + 1) * 512)
+    int base = 0;
+    for (RP c0 = a0; c0 < a1; c0 += 64) {
+      const int ne = (int)((a1 - c0) < (RP)64 ? (a1 - c0) : (RP)64);
+      int len = 0;
+      if (lane < ne) {
+        const int kk = tile.a_col[c0 + lane];
+        const RP st = tile.b_rp[kk];
+        len = (int)(tile.b_rp[kk + 1] - st);
+        s_st[w][lane] = st;
+        if (NUM) s_av[w][lane] = tile.a_val[c0 + lane];
+      }
+      int inc = len;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+      }
+      const int total = __shfl(inc, 63, 64);
+      if (lane < ne) s_pre[w][lane] = inc - len;
+      if (lane == 0) s_pre[w][ne] = total;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int x = w * kEscChunk + k * 64 + lane;
+        const int t = x - base;
+        if (t >= 0 && t < total) {
+          int lo = 0, hi = ne;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_pre[w][mid] <= t) lo = mid; else hi = mid;
+          }
+          const RP q = s_st[w][lo] + (RP)(t - s_pre[w][lo]);
+          key[k] = ((unsigned)tile.b_col[q] << kEscKeyShift) | (unsigned)x;
+          if (NUM) s_val[x] = (double)s_av[w][lo] * (double)tile.b_val[q];
+        }
+      }
+      base += total;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    esc_sort<K>(key, lane);
+#pragma unroll
+    for (int k = 0; k < K; k++) s_key[0][w * kEscChunk + k * 64 + lane] = key[k];
+    __syncthreads();
+    int src = 0;
+#pragma unroll
+    for (int L = kEscChunk; L < N; L <<= 1) {
+      for (int p = threadIdx.x; p < N; p += 64 * W) {
+        const unsigned kv = s_key[src][p];
+        const int run = p / L, pos = p - run * L;
+        const int other = (run ^ 1) * L;
+        const bool left = (run & 1) == 0;
+        int lo = 0, hi = L;  // left run: partner keys < kv; right run: partner keys <= kv
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          const unsigned o = s_key[src][other + mid];
+          if (o < kv || (!left && o == kv)) lo = mid + 1; else hi = mid;
+        }
+        s_key[src ^ 1][(run & ~1) * L + pos + lo] = kv;
+      }
+      __syncthreads();
+      src ^= 1;
+    }
+    // compress: wavefront w owns sorted positions [w * 512, (w + 1) * 512), as k * 64 + lane
+    int wcnt = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int p = w * kEscChunk + k * 64 + lane;
+      key[k] = s_key[src][p];
+      const unsigned prev = p > 0 ? s_key[src][p - 1] : kEscPad;
+      const bool head = key[k] != kEscPad && (p == 0 || (prev >> kEscKeyShift) != (key[k] >> kEscKeyShift));
+      wcnt += __popcll(__ballot(head));
+    }
+    if (lane == 0) s_cnt[w] = wcnt;
+    __syncthreads();
+    int before = 0, cnt = 0;
+#pragma unroll
+    for (int q = 0; q < W; q++) {
+      if (q < w) before += s_cnt[q];
+      cnt += s_cnt[q];
+    }
+    if (MODE == 0) {
+      if (threadIdx.x == 0) R.nnz[fr] = cnt;
+      __syncthreads();
+      continue;
+    }
+    if (threadIdx.x == 0) {
+      long long o;
+      if (MODE == 1 || (MODE == 2 && cnt <= R.cap[fr])) {
+        o = R.off[fr];
+      } else {
+        o = (long long)atomicAdd(R.cursor, (unsigned long long)cnt);
+        R.off[fr] = o;
+      }
+      if (o + cnt > R.out_limit) {
+        R.stats[3] = 1ull;
+        o = -1;
+      }
+      R.nnz[fr] = cnt;
+      s_o = o;
+    }
+    __syncthreads();
+    const long long o = s_o;
+    if (o >= 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int p = w * kEscChunk + k * 64 + lane;
+        const unsigned prev = p > 0 ? s_key[src][p - 1] : kEscPad;
+        const bool head = key[k] != kEscPad && (p == 0 || (prev >> kEscKeyShift) != (key[k] >> kEscKeyShift));
+        const unsigned long long mask = __ballot(head);
+        if (head) {
+          const unsigned col = key[k] >> kEscKeyShift;
+          double sum = s_val[key[k] & kEscIdxMask];
+          for (int p2 = p + 1; p2 < N; p2++) {
+            const unsigned kp = s_key[src][p2];
+            if (kp == kEscPad || (kp >> kEscKeyShift) != col) break;
+            sum += s_val[kp & kEscIdxMask];
+          }
+          const long long pos = o + before + __popcll(mask & ((1ull << lane) - 1ull));
+          tile.c_col[pos] = (int)col;
+          tile.c_val[pos] = (T)sum;
+        }
+        before += __popcll(mask);
       }
     }
     __syncthreads();

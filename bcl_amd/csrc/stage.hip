@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void k_ovf_bin(RowsDev<T, RP> R) {
     const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
     const long long prod = R.prod[fr];
     const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
-    const int b = bin_for(ub, prod, R.tn);
+    const int b = bin_for(ub, prod, R.tn, tile.n <= (1 << kEscColBits));
     const int pos = atomicAdd(&R.bin_count[b], 1);
     R.bin_items[(int64_t)b * R.bin_stride + pos] = fr;
   }
@@ -114,6 +114,8 @@ struct StagePlan : StageBase {
       tn.work_per_lane = std::max(1, opts.h_tuning[0]);
       tn.heavy_need = std::min(std::max(64, opts.h_tuning[1]), 4096);
       tn.load_shift = std::min(std::max(1, opts.h_tuning[2]), 2);
+      tn.esc_min = std::max(0, opts.h_tuning[3]);
+      tn.esc_max = std::min(std::max(0, opts.h_tuning[4]), kEscMaxProducts);
     }
     std::vector<TileDev<T, RP>> tiles;
     for (int p = 0; p < np; p++) tiles.push_back(make_tile<T, RP>(&A[p], &B[p]));
@@ -182,6 +184,10 @@ struct StagePlan : StageBase {
     fork(s);
     int q = 0;
     if (counts[kHeavyBin] > 0) F.run_bin(side[q++ % side.size()], mode_k, kHeavyBin, persistent ? -1 : counts[kHeavyBin]);
+    for (int b = kEscBin0 + kEscBins - 1; b >= kEscBin0; b--) {
+      if (counts[b] == 0) continue;
+      F.run_bin(side[q++ % side.size()], mode_k, b, counts[b]);
+    }
     for (int b = 9; b >= 1; b--) {
       if (counts[b] == 0) continue;
       F.run_bin(side[q++ % side.size()], mode_k, b, counts[b]);

@@ -15,6 +15,14 @@ from .csr import CSR
 from .device import DeviceCSR, Handle
 
 PLAIN, LEARNED, BATCHED_EXACT = 0, 1, 2
+# BinTuning defaults (bclx_stage_opts.h_tuning): work_per_lane, heavy_need, load_shift, esc_min, esc_max
+TUNING_DEFAULTS = (64, 4096, 1, 32, 4096)
+
+
+def full_tuning(t) -> np.ndarray:
+    """A complete BCLX_TUNING_LEN table: the given leading knobs, defaults for the rest."""
+    t = [int(x) for x in t]
+    return np.array(t + list(TUNING_DEFAULTS[len(t):]), np.int32)
 STAT_NAMES = ["products", "nnz", "overflow_rows", "predicted_rows", "capacity", "rows", "heavy_rows"]
 
 
@@ -36,7 +44,7 @@ class StagePlan:
         self._params = None if params is None else np.ascontiguousarray(params, np.float32)
         o.h_mlp_params = self._params.ctypes.data_as(ctypes.c_void_p) if self._params is not None else None
         o.persistent_blocks = 0
-        self._tuning = None if tuning is None else np.ascontiguousarray(tuning, np.int32)
+        self._tuning = None if tuning is None else np.ascontiguousarray(full_tuning(tuning), np.int32)
         o.h_tuning = self._tuning.ctypes.data_as(ctypes.c_void_p) if self._tuning is not None else None
         p = ctypes.c_void_p()
         call("bclx_stage_plan_create", h.h, self.npairs, da, db, ctypes.byref(o), ctypes.byref(p))

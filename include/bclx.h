@@ -62,6 +62,8 @@ typedef struct bclx_handle bclx_handle;
 typedef struct bclx_spgemm_plan bclx_spgemm_plan;
 typedef struct bclx_stage_plan bclx_stage_plan;
 
+#define BCLX_TUNING_LEN 5
+
 /* Last error message of the calling thread (empty string when none). */
 BCLX_API const char* bclx_last_error(void);
 
@@ -100,7 +102,12 @@ typedef struct {
   int32_t slack;
   const float* h_mlp_params; /* 5313 fp32 MLP parameters (NULL: mode 1 falls back to upper bounds) */
   int32_t persistent_blocks; /* 0 = automatic */
-  const int32_t* h_tuning;   /* optional autotuned table (see bclx_tuning_size); NULL = defaults */
+  const int32_t* h_tuning;   /* optional BCLX_TUNING_LEN binning knobs (autotuned), NULL = defaults:
+                              *   [0] work_per_lane (64)  lanes per row >= products / work_per_lane
+                              *   [1] heavy_need (4096)   rows needing more slots use the heavy accumulator
+                              *   [2] load_shift (1)      hash slots >= need << load_shift
+                              *   [3] esc_min (32)        rows with esc_min < products <= esc_max are sorted
+                              *   [4] esc_max (4096)      instead of hashed (0 turns sorting off; max 4096) */
 } bclx_stage_opts;
 
 BCLX_API bclx_status bclx_stage_plan_create(bclx_handle* h, int npairs, const bclx_csr* A, const bclx_csr* B,

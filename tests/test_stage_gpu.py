@@ -90,6 +90,20 @@ def test_stage_modes(handle, stage14, mode, pkind, graph):
     plan.close()
 
 
+@pytest.mark.parametrize("tuning", [(64, 4096, 1, 32, 0), (16, 1024, 2, 0, 4096), (256, 2048, 1, 64, 256),
+                                    (64, 4096, 1, 32, 1024)])
+def test_stage_tuning_choices(handle, stage14, tuning):
+    """Every binning choice (sorting off, all sortable rows sorted, narrow groups) gives the oracle's result."""
+    print(f"[stage] tuning={tuning}", flush=True)
+    A, B, pairs = stage_tiles(stage14, stage14, 2, 1)
+    for mode in (BATCHED_EXACT, LEARNED):
+        plan = StagePlan(handle, A, B, mode=mode, params=predictor.init_params(seed=3), tuning=np.array(tuning))
+        plan.execute()
+        torch.cuda.synchronize()
+        check_stage(plan, pairs)
+        plan.close()
+
+
 def test_stage_uneven_grid_learned(handle, stage13):
     A, B, pairs = stage_tiles(stage13, stage13, 4, 1)
     for mode in (BATCHED_EXACT, LEARNED):
@@ -139,6 +153,7 @@ def test_predictor_gain(handle, kind):
     Without the network every row's capacity is the exact bound min(products, n); the exact pipeline runs the
     symbolic pass instead. All three must give the oracle's result. The trained network must size the slot region
     well below the bound (its point: smaller tables and output arena) with few overflows; the times are reported.
+    Sorted (ESC) rows are exact in one pass and need no prediction, so sorting is turned off here: every row hashes.
     """
     params = predictor.load()
     if params is None:
@@ -154,7 +169,7 @@ def test_predictor_gain(handle, kind):
     res = {}
     for name, mode, prm in (("trained", LEARNED, params), ("bound", LEARNED, None), ("exact", BATCHED_EXACT, None)):
         print(f"[gain] {kind} {name}", flush=True)
-        plan = StagePlan(handle, A, B, mode=mode, params=prm)
+        plan = StagePlan(handle, A, B, mode=mode, params=prm, tuning=np.array([64, 4096, 1, 32, 0]))
         ms = _median_ms(handle, plan)
         st = plan.stats()
         check_stage(plan, pairs)
