@@ -247,8 +247,8 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
     return;
   }
   if (b == kHeavyBin && (max_n + 31) / 32 <= kHRWords) {
-    const int nslots = (int)std::max<int64_t>(1, std::min<int64_t>(count_hint >= 0 ? count_hint : h->num_cus,
-                                                                     (int64_t)h->num_cus));
+    const int64_t resident = (int64_t)kHRResident * h->num_cus;
+    const int nslots = (int)std::max<int64_t>(1, std::min<int64_t>(count_hint >= 0 ? count_hint : resident, resident));
     ensure_dense_slots(h, nslots, (size_t)max_n);
     double* acc = h->dense_slots.as<double>();
     const int64_t sd = (int64_t)h->dense_slot_doubles;
@@ -331,8 +331,8 @@ int64_t FlatRows<T, RP>::read_stat(int q) {
 
 template <typename T, typename RP>
 void FlatRows<T, RP>::prepare_heavy() {
-  // rank-compact kernel: one scratch of max_n doubles per workgroup; global dense fallback: the same shape
-  ensure_dense_slots(h, h->num_cus, (size_t)max_n);
+  // rank-compact kernel: one scratch of max_n doubles per resident workgroup; global dense fallback: the same shape
+  ensure_dense_slots(h, kHRResident * h->num_cus, (size_t)max_n);
 }
 
 template struct FlatRows<float, int32_t>;

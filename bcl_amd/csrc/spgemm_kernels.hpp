@@ -810,21 +810,22 @@ __global__ __launch_bounds__(64 * kEscWaves) void k_esc(RowsDev<T, RP> R, int bi
 }
 
 // Sort-based rows of 512 * W products or fewer, W wavefronts per row: each wavefront expands and sorts 512 of the
-// row's products in registers (as the single-wavefront kernel does), the sorted runs are merged pairwise in LDS (each
-// key's merged position is its own index plus its rank in the partner run, found by binary search; ties go to the
-// left run, so the pad keys fill the tail), and the block compresses the sorted row. LDS: 16 bytes per product.
+// row's products in registers (as the single-wavefront kernel does), the sorted runs are merged pairwise in place in
+// LDS (each key's merged position is its own index plus its rank in the partner run, found by binary search while
+// every key is still in place; ties go to the left run, so the pad keys fill the tail), and the block compresses the
+// sorted row. LDS: 12 bytes per product (key and float64 product) plus one staged chunk of A entries.
 constexpr int kEscChunk = 512;
 
 template <int W, int MODE, typename T, typename RP>
-__global__ __launch_bounds__(64 * W) void k_esc_block(RowsDev<T, RP> R, int bin) {
+__global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int bin) {
   constexpr int N = kEscChunk * W;
   constexpr int K = kEscChunk / 64;
   constexpr bool NUM = MODE != 0;
-  __shared__ unsigned s_key[2][N];
+  __shared__ unsigned s_key[N];
   __shared__ double s_val[NUM ? N : 1];
-  __shared__ int s_pre[W][65];
-  __shared__ RP s_st[W][64];
-  __shared__ T s_av[W][64];
+  __shared__ int s_pre[65];
+  __shared__ RP s_st[64];
+  __shared__ T s_av[64];
   __shared__ int s_cnt[W];
   __shared__ long long s_o;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -838,30 +839,30 @@ __global__ __launch_bounds__(64 * W) void k_esc_block(RowsDev<T, RP> R, int bin)
     unsigned key[K];
 #pragma unroll
     for (int k = 0; k < K; k++) key[k] = kEscPad;
-    // expand this wavefront's products x in [w * 512, (w + 1) * 512)
+    // expand: wavefront 0 stages each chunk of 64 A entries; wavefront w takes products x in [w * 512, (w + 1) * 512)
     int base = 0;
     for (RP c0 = a0; c0 < a1; c0 += 64) {
       const int ne = (int)((a1 - c0) < (RP)64 ? (a1 - c0) : (RP)64);
-      int len = 0;
-      if (lane < ne) {
-        const int kk = tile.a_col[c0 + lane];
-        const RP st = tile.b_rp[kk];
-        len = (int)(tile.b_rp[kk + 1] - st);
-        s_st[w][lane] = st;
-        if (NUM) s_av[w][lane] = tile.a_val[c0 + lane];
-      }
-      int inc = len;
+      if (w == 0) {
+        int len = 0;
+        if (lane < ne) {
+          const int kk = tile.a_col[c0 + lane];
+          const RP st = tile.b_rp[kk];
+          len = (int)(tile.b_rp[kk + 1] - st);
+          s_st[lane] = st;
+          if (NUM) s_av[lane] = tile.a_val[c0 + lane];
+        }
+        int inc = len;
 #pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += t;
+        for (int d = 1; d < 64; d <<= 1) {
+          const int t = __shfl_up(inc, d, 64);
+          if (lane >= d) inc += t;
+        }
+        if (lane < ne) s_pre[lane] = inc - len;
+        if (lane == 63) s_pre[ne] = inc;
       }
-      const int total = __shfl(inc, 63, 64);
-      if (lane < ne) s_pre[w][lane] = inc - len;
-      if (lane == 0) s_pre[w][ne] = total;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __syncthreads();
+      const int total = s_pre[ne];
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int x = w * kEscChunk + k * 64 + lane;
@@ -870,48 +871,51 @@ __global__ __launch_bounds__(64 * W) void k_esc_block(RowsDev<T, RP> R, int bin)
           int lo = 0, hi = ne;
           while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (s_pre[w][mid] <= t) lo = mid; else hi = mid;
+            if (s_pre[mid] <= t) lo = mid; else hi = mid;
           }
-          const RP q = s_st[w][lo] + (RP)(t - s_pre[w][lo]);
+          const RP q = s_st[lo] + (RP)(t - s_pre[lo]);
           key[k] = ((unsigned)tile.b_col[q] << kEscKeyShift) | (unsigned)x;
-          if (NUM) s_val[x] = (double)s_av[w][lo] * (double)tile.b_val[q];
+          if (NUM) s_val[x] = (double)s_av[lo] * (double)tile.b_val[q];
         }
       }
       base += total;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __syncthreads();
     }
     esc_sort<K>(key, lane);
+    // thread (w, lane) owns positions w * 512 + k * 64 + lane in every round
 #pragma unroll
-    for (int k = 0; k < K; k++) s_key[0][w * kEscChunk + k * 64 + lane] = key[k];
+    for (int k = 0; k < K; k++) s_key[w * kEscChunk + k * 64 + lane] = key[k];
     __syncthreads();
-    int src = 0;
 #pragma unroll
     for (int L = kEscChunk; L < N; L <<= 1) {
-      for (int p = threadIdx.x; p < N; p += 64 * W) {
-        const unsigned kv = s_key[src][p];
+      int dst[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const int p = w * kEscChunk + k * 64 + lane;
         const int run = p / L, pos = p - run * L;
         const int other = (run ^ 1) * L;
         const bool left = (run & 1) == 0;
-        int lo = 0, hi = L;  // left run: partner keys < kv; right run: partner keys <= kv
+        int lo = 0, hi = L;  // left run: partner keys < key; right run: partner keys <= key
         while (lo < hi) {
           const int mid = (lo + hi) >> 1;
-          const unsigned o = s_key[src][other + mid];
-          if (o < kv || (!left && o == kv)) lo = mid + 1; else hi = mid;
+          const unsigned o = s_key[other + mid];
+          if (o < key[k] || (!left && o == key[k])) lo = mid + 1; else hi = mid;
         }
-        s_key[src ^ 1][(run & ~1) * L + pos + lo] = kv;
+        dst[k] = (run & ~1) * L + pos + lo;
       }
       __syncthreads();
-      src ^= 1;
+#pragma unroll
+      for (int k = 0; k < K; k++) s_key[dst[k]] = key[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < K; k++) key[k] = s_key[w * kEscChunk + k * 64 + lane];
     }
     // compress: wavefront w owns sorted positions [w * 512, (w + 1) * 512), as k * 64 + lane
     int wcnt = 0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const int p = w * kEscChunk + k * 64 + lane;
-      key[k] = s_key[src][p];
-      const unsigned prev = p > 0 ? s_key[src][p - 1] : kEscPad;
+      const unsigned prev = p > 0 ? s_key[p - 1] : kEscPad;
       const bool head = key[k] != kEscPad && (p == 0 || (prev >> kEscKeyShift) != (key[k] >> kEscKeyShift));
       wcnt += __popcll(__ballot(head));
     }
@@ -949,14 +953,14 @@ __global__ __launch_bounds__(64 * W) void k_esc_block(RowsDev<T, RP> R, int bin)
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int p = w * kEscChunk + k * 64 + lane;
-        const unsigned prev = p > 0 ? s_key[src][p - 1] : kEscPad;
+        const unsigned prev = p > 0 ? s_key[p - 1] : kEscPad;
         const bool head = key[k] != kEscPad && (p == 0 || (prev >> kEscKeyShift) != (key[k] >> kEscKeyShift));
         const unsigned long long mask = __ballot(head);
         if (head) {
           const unsigned col = key[k] >> kEscKeyShift;
           double sum = s_val[key[k] & kEscIdxMask];
           for (int p2 = p + 1; p2 < N; p2++) {
-            const unsigned kp = s_key[src][p2];
+            const unsigned kp = s_key[p2];
             if (kp == kEscPad || (kp >> kEscKeyShift) != col) break;
             sum += s_val[kp & kEscIdxMask];
           }
@@ -982,14 +986,33 @@ __global__ __launch_bounds__(64 * W) void k_esc_block(RowsDev<T, RP> R, int bin)
 // 3 bump-allocate. Needs n <= 32 * kHRWords; wider tiles use k_heavy (global dense accumulator).
 // ------------------------------------------------------------------------------------------------------------
 
-constexpr int kHRBlock = 1024;
-constexpr int kHRWords = 16384;
+constexpr int kHRBlock = 512;    // two workgroups per CU (LDS: 64 KB bitmap + group prefixes + staging)
+constexpr int kHRResident = 2;   // resident heavy-row workgroups per CU (one accumulator scratch each)
+constexpr int kHRWords = 16384;  // bitmap words: tiles up to 2^19 columns
+constexpr int kHRGroup = 16;     // words per prefix group
+
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+
+// Output rank of column j: set bits before j. The group prefix counts the bits before j's 16-word group; the words of
+// the group before j's word are read as four 16-byte LDS loads.
+__device__ inline int hr_rank(const unsigned* bits, const int* gpre, int j) {
+  const int w = j >> 5, g = w / kHRGroup, wi = w % kHRGroup;
+  const u32x4_t* q = reinterpret_cast<const u32x4_t*>(bits + g * kHRGroup);
+  int r = gpre[g];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const u32x4_t x = q[v];
+    r += (4 * v + 0 < wi ? __popc(x.x) : 0) + (4 * v + 1 < wi ? __popc(x.y) : 0) + (4 * v + 2 < wi ? __popc(x.z) : 0) +
+         (4 * v + 3 < wi ? __popc(x.w) : 0);
+  }
+  return r + __popc(bits[w] & ((1u << (j & 31)) - 1u));
+}
 
 template <int MODE, typename T, typename RP>
 __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int bin, double* scratch,
                                                          int64_t scratch_stride) {
-  __shared__ unsigned s_bits[kHRWords];
-  __shared__ int s_pre[kHRWords];
+  __shared__ __align__(16) unsigned s_bits[kHRWords];
+  __shared__ int s_gpre[kHRWords / kHRGroup];
   __shared__ long long s_scan[kHRBlock / 64 + 1];
   __shared__ long long s_base;
   __shared__ RP s_est[kHRBlock];
@@ -997,6 +1020,7 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
   __shared__ T s_eval[kHRBlock];
   __shared__ long long s_lscan[kHRBlock / 64 + 1];
   constexpr bool NUM = MODE != 0;
+  constexpr int kGroupsPerThread = kHRWords / kHRGroup / kHRBlock;  // 2
   const int tid = threadIdx.x;
   double* acc = scratch + (int64_t)blockIdx.x * scratch_stride;
   const int count = R.bin_count[bin];
@@ -1006,22 +1030,40 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
     const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
     const int64_t i = fr - tile.flat0;
     const int W = (tile.n + 31) / 32;
-    for (int w = tid; w < W; w += kHRBlock) s_bits[w] = 0u;
+    const int Wr = (W + kHRGroup - 1) / kHRGroup * kHRGroup;  // whole groups (the tail words stay zero)
+    for (int w = tid; w < Wr; w += kHRBlock) s_bits[w] = 0u;
     __syncthreads();
     const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
     for_each_product<kHRBlock, kHRBlock, false, T, RP>(
         tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
         [&](int j, double) { atomicOr(&s_bits[j >> 5], 1u << (j & 31)); });
     __syncthreads();
-    const int per = (W + kHRBlock - 1) / kHRBlock;
-    const int w0 = tid * per, w1 = min(W, w0 + per);
+    // group prefixes: thread t counts groups [t * 2, t * 2 + 2), in order
+    const int ng = Wr / kHRGroup;
+    int gc[kGroupsPerThread];
     long long c = 0;
-    for (int w = w0; w < w1; w++) c += __popc(s_bits[w]);
+#pragma unroll
+    for (int q = 0; q < kGroupsPerThread; q++) {
+      const int g = tid * kGroupsPerThread + q;
+      int n1 = 0;
+      if (g < ng) {
+        const u32x4_t* p4 = reinterpret_cast<const u32x4_t*>(s_bits + g * kHRGroup);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const u32x4_t x = p4[v];
+          n1 += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+        }
+      }
+      gc[q] = n1;
+      c += n1;
+    }
     long long total;
     long long run = block_excl_scan<kHRBlock, long long>(c, &total, s_scan);
-    for (int w = w0; w < w1; w++) {
-      s_pre[w] = (int)run;
-      run += __popc(s_bits[w]);
+#pragma unroll
+    for (int q = 0; q < kGroupsPerThread; q++) {
+      const int g = tid * kGroupsPerThread + q;
+      if (g < ng) s_gpre[g] = (int)run;
+      run += gc[q];
     }
     if (MODE == 0) {
       if (tid == 0) {
@@ -1053,9 +1095,8 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
     // barrier order every add before the read-and-reset below; no device-scope fence (an L2 write-back) is needed.
     for_each_product<kHRBlock, kHRBlock, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
                                                        [&](int j, double v) {
-                                                         const int w = j >> 5;
-                                                         const int r = s_pre[w] + __popc(s_bits[w] & ((1u << (j & 31)) - 1u));
-                                                         __hip_atomic_fetch_add(&acc[r], v, __ATOMIC_RELAXED,
+                                                         __hip_atomic_fetch_add(&acc[hr_rank(s_bits, s_gpre, j)], v,
+                                                                                __ATOMIC_RELAXED,
                                                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
                                                        });
     __syncthreads();
@@ -1071,7 +1112,8 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
     if (base >= 0)
       for (int w = tid; w < W; w += kHRBlock) {
         unsigned word = s_bits[w];
-        long long r = base + s_pre[w];
+        if (!word) continue;
+        long long r = base + hr_rank(s_bits, s_gpre, w * 32);
         while (word) {
           tile.c_col[r++] = w * 32 + (__ffs(word) - 1);
           word &= word - 1;

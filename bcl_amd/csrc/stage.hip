@@ -352,7 +352,7 @@ struct StagePlan : StageBase {
     final_off.alloc(sizeof(int64_t) * (F.nflat + 1));
     tile_rp.alloc(sizeof(int64_t) * (F.nflat + npairs + 1));
     // heavy-row accumulators are allocated before capture
-    ensure_dense_slots(h, h->num_cus, (size_t)F.max_n);
+    ensure_dense_slots(h, kHRResident * h->num_cus, (size_t)F.max_n);
     sizing_run();
     if (opts.use_graph) {
       // capture on a plan-owned stream (the handle's stream may be the legacy default stream, which cannot be
