@@ -247,16 +247,13 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
     return;
   }
   if (b == kHeavyBin && (max_n + 31) / 32 <= kHRWords) {
-    const int64_t resident = (int64_t)kHRResident * h->num_cus;
-    const int nslots = (int)std::max<int64_t>(1, std::min<int64_t>(count_hint >= 0 ? count_hint : resident, resident));
-    ensure_dense_slots(h, nslots, (size_t)max_n);
-    double* acc = h->dense_slots.as<double>();
-    const int64_t sd = (int64_t)h->dense_slot_doubles;
+    const int64_t resident = (mode == 0 ? 2 : 1) * (int64_t)h->num_cus;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(count_hint >= 0 ? count_hint : resident, resident));
     switch (mode) {
-      case 0: hipLaunchKernelGGL((k_heavy_rank<0, T, RP>), dim3(nslots), dim3(kHRBlock), 0, s, R, kHeavyBin, acc, sd); break;
-      case 1: hipLaunchKernelGGL((k_heavy_rank<1, T, RP>), dim3(nslots), dim3(kHRBlock), 0, s, R, kHeavyBin, acc, sd); break;
-      case 2: hipLaunchKernelGGL((k_heavy_rank<2, T, RP>), dim3(nslots), dim3(kHRBlock), 0, s, R, kHeavyBin, acc, sd); break;
-      default: hipLaunchKernelGGL((k_heavy_rank<3, T, RP>), dim3(nslots), dim3(kHRBlock), 0, s, R, kHeavyBin, acc, sd); break;
+      case 0: hipLaunchKernelGGL((k_heavy_rank<0, T, RP>), dim3(grid), dim3(hr_block<0>()), 0, s, R, kHeavyBin); break;
+      case 1: hipLaunchKernelGGL((k_heavy_rank<1, T, RP>), dim3(grid), dim3(hr_block<1>()), 0, s, R, kHeavyBin); break;
+      case 2: hipLaunchKernelGGL((k_heavy_rank<2, T, RP>), dim3(grid), dim3(hr_block<2>()), 0, s, R, kHeavyBin); break;
+      default: hipLaunchKernelGGL((k_heavy_rank<3, T, RP>), dim3(grid), dim3(hr_block<3>()), 0, s, R, kHeavyBin); break;
     }
     BCLX_HIP(hipGetLastError());
     return;
@@ -331,8 +328,9 @@ int64_t FlatRows<T, RP>::read_stat(int q) {
 
 template <typename T, typename RP>
 void FlatRows<T, RP>::prepare_heavy() {
-  // rank-compact kernel: one scratch of max_n doubles per resident workgroup; global dense fallback: the same shape
-  ensure_dense_slots(h, kHRResident * h->num_cus, (size_t)max_n);
+  // the rank-slice kernel keeps everything in LDS; the global dense fallback (tiles wider than 2^19 columns) needs
+  // one accumulator of max_n doubles per resident workgroup
+  if ((max_n + 31) / 32 > kHRWords) ensure_dense_slots(h, h->num_cus, (size_t)max_n);
 }
 
 template struct FlatRows<float, int32_t>;

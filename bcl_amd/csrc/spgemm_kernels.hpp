@@ -976,20 +976,25 @@ __global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int b
 }
 
 // ------------------------------------------------------------------------------------------------------------
-// Heavy rows (more output entries than the largest LDS table), rank-compact accumulator: one workgroup per row.
-//   pass 1: LDS bitmap of the row's columns (one atomicOr per product) -> per-word prefix popcounts, so every
-//           column's output rank is known and nnz(C_i) is exact;
-//   pass 2 (numeric modes): each product is added, in float64, into a per-workgroup scratch indexed by rank, i.e. a
-//           compact array of nnz(C_i) doubles that stays in L2 (a column-indexed dense array would be n doubles);
-//   emission walks the bitmap in column order and writes (column, value) at the row's offset + rank.
+// Heavy rows (more products than the sort-based bins take and more output entries than the largest hash table),
+// rank-slice accumulator: one workgroup per row, everything in LDS.
+//   pass 1: LDS bitmap of the row's columns (one atomicOr per product) and prefix counts per 16-word group, so every
+//           column's output rank is known and nnz(C_i) is exact; the columns are written straight from the bitmap;
+//   pass 2 (numeric modes), per slice of hr_slice() ranks: the products whose columns fall in the slice's column range
+//           are added in float64 into an LDS array indexed by rank - slice start (ds_add_f64), then written out with
+//           unit stride. Most rows fit one slice; larger rows enumerate their products once per slice.
 // Modes as in k_hash: 0 symbolic, 1 numeric at off[fr], 2 predicted (slot if nnz <= cap, else bump-allocate),
 // 3 bump-allocate. Needs n <= 32 * kHRWords; wider tiles use k_heavy (global dense accumulator).
 // ------------------------------------------------------------------------------------------------------------
 
-constexpr int kHRBlock = 512;    // two workgroups per CU (LDS: 64 KB bitmap + group prefixes + staging)
-constexpr int kHRResident = 2;   // resident heavy-row workgroups per CU (one accumulator scratch each)
 constexpr int kHRWords = 16384;  // bitmap words: tiles up to 2^19 columns
 constexpr int kHRGroup = 16;     // words per prefix group
+// float64 accumulators per slice: what LDS holds beside the bitmap and the entry staging (72 KB or 64 KB)
+template <typename T, typename RP>
+constexpr int hr_slice() { return sizeof(T) == 4 && sizeof(RP) == 4 ? 9216 : 8192; }
+// symbolic: 512 threads, two workgroups per CU; numeric: 1024 threads, one workgroup per CU (bitmap + slice)
+template <int MODE>
+constexpr int hr_block() { return MODE == 0 ? 512 : 1024; }
 
 typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
 
@@ -1008,21 +1013,43 @@ __device__ inline int hr_rank(const unsigned* bits, const int* gpre, int j) {
   return r + __popc(bits[w] & ((1u << (j & 31)) - 1u));
 }
 
+// Column of output rank r (0 <= r < number of set bits): the r-th set bit. One thread.
+__device__ inline int hr_select(const unsigned* bits, const int* gpre, int ng, int r) {
+  int lo = 0, hi = ng;  // last group with gpre <= r
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (gpre[mid] <= r) lo = mid; else hi = mid;
+  }
+  int left = r - gpre[lo];
+  int w = lo * kHRGroup;
+  while (true) {
+    const int c = __popc(bits[w]);
+    if (left < c) break;
+    left -= c;
+    w++;
+  }
+  unsigned word = bits[w];
+  for (int q = 0; q < left; q++) word &= word - 1;
+  return w * 32 + (__ffs(word) - 1);
+}
+
 template <int MODE, typename T, typename RP>
-__global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int bin, double* scratch,
-                                                         int64_t scratch_stride) {
+__global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> R, int bin) {
+  constexpr int BLOCK = hr_block<MODE>();
+  constexpr bool NUM = MODE != 0;
   __shared__ __align__(16) unsigned s_bits[kHRWords];
   __shared__ int s_gpre[kHRWords / kHRGroup];
-  __shared__ long long s_scan[kHRBlock / 64 + 1];
+  constexpr int kHRSlice = hr_slice<T, RP>();
+  __shared__ double s_acc[NUM ? kHRSlice : 1];
+  __shared__ long long s_scan[BLOCK / 64 + 1];
   __shared__ long long s_base;
-  __shared__ RP s_est[kHRBlock];
-  __shared__ long long s_epre[kHRBlock + 1];
-  __shared__ T s_eval[kHRBlock];
-  __shared__ long long s_lscan[kHRBlock / 64 + 1];
-  constexpr bool NUM = MODE != 0;
-  constexpr int kGroupsPerThread = kHRWords / kHRGroup / kHRBlock;  // 2
+  __shared__ int s_crange[2];
+  __shared__ RP s_est[BLOCK];
+  __shared__ long long s_epre[BLOCK + 1];
+  __shared__ T s_eval[BLOCK];
+  __shared__ long long s_lscan[BLOCK / 64 + 1];
+  constexpr int kGroupsPerThread = (kHRWords / kHRGroup + BLOCK - 1) / BLOCK;
   const int tid = threadIdx.x;
-  double* acc = scratch + (int64_t)blockIdx.x * scratch_stride;
   const int count = R.bin_count[bin];
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
   for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
@@ -1031,14 +1058,13 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
     const int64_t i = fr - tile.flat0;
     const int W = (tile.n + 31) / 32;
     const int Wr = (W + kHRGroup - 1) / kHRGroup * kHRGroup;  // whole groups (the tail words stay zero)
-    for (int w = tid; w < Wr; w += kHRBlock) s_bits[w] = 0u;
+    for (int w = tid; w < Wr; w += BLOCK) s_bits[w] = 0u;
     __syncthreads();
     const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
-    for_each_product<kHRBlock, kHRBlock, false, T, RP>(
-        tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
-        [&](int j, double) { atomicOr(&s_bits[j >> 5], 1u << (j & 31)); });
+    for_each_product<BLOCK, BLOCK, false, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
+                                                 [&](int j, double) { atomicOr(&s_bits[j >> 5], 1u << (j & 31)); });
     __syncthreads();
-    // group prefixes: thread t counts groups [t * 2, t * 2 + 2), in order
+    // group prefixes, in order: thread t counts groups [t * kGroupsPerThread, (t + 1) * kGroupsPerThread)
     const int ng = Wr / kHRGroup;
     int gc[kGroupsPerThread];
     long long c = 0;
@@ -1058,7 +1084,7 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
       c += n1;
     }
     long long total;
-    long long run = block_excl_scan<kHRBlock, long long>(c, &total, s_scan);
+    long long run = block_excl_scan<BLOCK, long long>(c, &total, s_scan);
 #pragma unroll
     for (int q = 0; q < kGroupsPerThread; q++) {
       const int g = tid * kGroupsPerThread + q;
@@ -1091,35 +1117,40 @@ __global__ __launch_bounds__(kHRBlock) void k_heavy_rank(RowsDev<T, RP> R, int b
       atomicAdd(&R.stats[1], 1ull);
     }
     __syncthreads();
-    // The scratch is private to this workgroup, so workgroup-scope atomics (performed in L2) and the workgroup
-    // barrier order every add before the read-and-reset below; no device-scope fence (an L2 write-back) is needed.
-    for_each_product<kHRBlock, kHRBlock, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
-                                                       [&](int j, double v) {
-                                                         __hip_atomic_fetch_add(&acc[hr_rank(s_bits, s_gpre, j)], v,
-                                                                                __ATOMIC_RELAXED,
-                                                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-                                                       });
-    __syncthreads();
     const long long base = s_base;
-    // values: the scratch is already in output (rank) order, so this pass is unit-stride; it also re-zeroes the
-    // scratch for the next row, whether or not this row was written
-    for (long long r = tid; r < total; r += kHRBlock) {
-      const double v = __hip_atomic_exchange(&acc[r], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (base >= 0) tile.c_val[base + r] = (T)v;
+    if (base < 0) {  // refused: nothing is written
+      __syncthreads();
+      continue;
     }
-    // columns: words dealt round-robin (a row's columns cluster at the R-MAT hubs' low ids, so contiguous word
-    // ranges per thread would leave one thread writing most of the row)
-    if (base >= 0)
-      for (int w = tid; w < W; w += kHRBlock) {
-        unsigned word = s_bits[w];
-        if (!word) continue;
-        long long r = base + hr_rank(s_bits, s_gpre, w * 32);
-        while (word) {
-          tile.c_col[r++] = w * 32 + (__ffs(word) - 1);
-          word &= word - 1;
-        }
+    // columns straight from the bitmap (words dealt round-robin: a row's columns cluster at the R-MAT hubs' ids)
+    for (int w = tid; w < W; w += BLOCK) {
+      unsigned word = s_bits[w];
+      if (!word) continue;
+      long long r = base + hr_rank(s_bits, s_gpre, w * 32);
+      while (word) {
+        tile.c_col[r++] = w * 32 + (__ffs(word) - 1);
+        word &= word - 1;
       }
-    __syncthreads();
+    }
+    // values, one rank slice at a time
+    for (long long s0 = 0; s0 < total; s0 += kHRSlice) {
+      const int len = (int)(total - s0 < kHRSlice ? total - s0 : kHRSlice);
+      for (int t = tid; t < len; t += BLOCK) s_acc[t] = 0.0;
+      if (tid == 0) {  // column range of the slice (the whole row when it is one slice)
+        s_crange[0] = s0 == 0 ? 0 : hr_select(s_bits, s_gpre, ng, (int)s0);
+        s_crange[1] = s0 + len >= total ? INT_MAX : hr_select(s_bits, s_gpre, ng, (int)(s0 + len));
+      }
+      __syncthreads();
+      const int c_lo = s_crange[0], c_hi = s_crange[1];
+      for_each_product<BLOCK, BLOCK, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
+                                                  [&](int j, double v) {
+                                                    if (j >= c_lo && j < c_hi)
+                                                      atomicAdd(&s_acc[hr_rank(s_bits, s_gpre, j) - (int)s0], v);
+                                                  });
+      __syncthreads();
+      for (int t = tid; t < len; t += BLOCK) tile.c_val[base + s0 + t] = (T)s_acc[t];
+      __syncthreads();
+    }
   }
 }
 

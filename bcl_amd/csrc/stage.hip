@@ -351,8 +351,8 @@ struct StagePlan : StageBase {
       BCLX_HIP(hipMemcpy(params.p, opts.h_mlp_params, sizeof(float) * kNParam, hipMemcpyHostToDevice));
     final_off.alloc(sizeof(int64_t) * (F.nflat + 1));
     tile_rp.alloc(sizeof(int64_t) * (F.nflat + npairs + 1));
-    // heavy-row accumulators are allocated before capture
-    ensure_dense_slots(h, kHRResident * h->num_cus, (size_t)F.max_n);
+    // heavy-row accumulators of tiles wider than the LDS bitmap are allocated before capture
+    if ((F.max_n + 31) / 32 > kHRWords) ensure_dense_slots(h, h->num_cus, (size_t)F.max_n);
     sizing_run();
     if (opts.use_graph) {
       // capture on a plan-owned stream (the handle's stream may be the legacy default stream, which cannot be
