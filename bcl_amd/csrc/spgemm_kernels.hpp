@@ -8,6 +8,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <type_traits>
 
 namespace bclx {
 
@@ -210,34 +211,32 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_
     if (tid == lt) prod = s;
   }
   R.prod[fr] = valid ? prod : 0;
-  {
-    const long long ub0 = prod < (long long)tile.n ? prod : (long long)tile.n;
-    const int b0 = valid ? bin_for(ub0, prod, R.tn, tile.n <= (1 << kEscColBits)) : 0;
-    if (b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
-    if (pred_min >= 0) {
-      // learned path: every row starts at the exact bound min(prod, n); rows with more than pred_min products that
-      // hash (sorted rows are exact in one pass and keep the bound) go on the list whose capacities the predictor
-      // then lowers
-      R.cap[fr] = valid ? (int32_t)ub0 : 0;
-      const bool pr = valid && prod > pred_min && b0 < kEscBin0;
-      int ppos = 0;
-      if (pr) ppos = atomicAdd(&s_npred, 1);
-      __syncthreads();
-      if (tid == 0 && s_npred > 0) s_pbase = atomicAdd(R.pred_count, s_npred);
-      __syncthreads();
-      if (pr) R.pred_items[s_pbase + ppos] = (int32_t)fr;
-    }
+  const long long ub0 = prod < (long long)tile.n ? prod : (long long)tile.n;
+  const int b0 = valid ? bin_for(ub0, prod, R.tn, tile.n <= (1 << kEscColBits)) : 0;
+  if (b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
+  bool pr = false;
+  if (pred_min >= 0) {
+    // learned path: every row starts at the exact bound min(prod, n); rows with more than pred_min products that
+    // hash (sorted rows are exact in one pass and keep the bound) go on the list whose capacities the predictor
+    // then lowers, and are binned after the prediction
+    R.cap[fr] = valid ? (int32_t)ub0 : 0;
+    pr = valid && prod > pred_min && b0 < kEscBin0;
+    int ppos = 0;
+    if (pr) ppos = atomicAdd(&s_npred, 1);
+    __syncthreads();
+    if (tid == 0 && s_npred > 0) s_pbase = atomicAdd(R.pred_count, s_npred);
+    __syncthreads();
+    if (pr) R.pred_items[s_pbase + ppos] = (int32_t)fr;
   }
   if (do_bin) {
-    const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
-    const int b = valid ? bin_for(ub, prod, R.tn, tile.n <= (1 << kEscColBits)) : 0;
+    const int b = pr ? 0 : b0;
     int pos = 0;
     if (b > 0) pos = atomicAdd(&s_bcount[b], 1);
     __syncthreads();
     if (tid > 0 && tid < kNumBins && s_bcount[tid] > 0) s_bbase[tid] = atomicAdd(&R.bin_count[tid], s_bcount[tid]);
     __syncthreads();
     if (b > 0) R.bin_items[b * R.bin_stride + s_bbase[b] + pos] = (int)fr;
-    if (b == 0) R.nnz[fr] = 0;  // empty and padding rows
+    if (b == 0 && !pr) R.nnz[fr] = 0;  // empty and padding rows
   }
   long long tot = block_sum<kRowBlock, long long>(prod, s_red);
   if (tid == 0 && tot) atomicAdd(&R.stats[0], (unsigned long long)tot);
@@ -309,11 +308,50 @@ __device__ inline long long group_excl_scan(long long v, long long* total, long 
   }
 }
 
+// The product loop of a staged chunk of A entries: entry e's products are B positions e_start[e] + 0 ..
+// e_pre[e + 1] - e_pre[e] - 1 (e_pre holds ne + 1 prefix counts, e_pre[ne] = total). Lane l takes products l,
+// l + G, ..., U at a time: the U entry lookups (binary searches over the prefixes) advance in lockstep and the U loads
+// are issued together, so their latencies overlap. f(j, v) per product (v = a * b in float64; 0 when !NUM).
+template <int G, bool NUM, int U, typename T, typename RP, typename F>
+__device__ inline void enumerate_staged(const TileDev<T, RP>& tile, int ne, long long total, int lane,
+                                        const RP* e_start, const long long* e_pre, const T* e_val,
+                                        const volatile int* stop, F&& f) {
+  int top = 1;  // largest power of two < ne (0 when ne == 1)
+  while (top < ne) top <<= 1;
+  top >>= 1;
+  for (long long t0 = lane; t0 < total; t0 += (long long)U * G) {
+    if (stop && *stop) break;
+    int lo[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) lo[u] = 0;
+    for (int step = top; step > 0; step >>= 1) {  // entry e with e_pre[e] <= t < e_pre[e + 1]
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int c = lo[u] + step;
+        if (c < ne && e_pre[c] <= t0 + (long long)u * G) lo[u] = c;
+      }
+    }
+    int j[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const long long t = t0 + (long long)u * G;
+      if (t < total) {
+        const RP q = e_start[lo[u]] + (RP)(t - e_pre[lo[u]]);
+        j[u] = tile.b_col[q];
+        v[u] = NUM ? (double)e_val[lo[u]] * (double)tile.b_val[q] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (t0 + (long long)u * G < total) f(j[u], v[u]);
+  }
+}
+
 // Load-balanced enumeration of the products of one row: the group stages the B-row ranges of up to G entries of
-// A's row in LDS, prefix-sums their lengths, and lane l takes products l, l + G, ... (entry found by binary search),
-// two at a time so their loads overlap. Calls f(j, v) per product (column j, value a * b in float64; v = 0 when
-// !NUM). `stop` (optional, LDS) ends the enumeration early once set.
-template <int G, int BLOCK, bool NUM, typename T, typename RP, typename F>
+// A's row in LDS, prefix-sums their lengths and runs enumerate_staged over them. `stop` (optional, LDS) ends the
+// enumeration early once set.
+template <int G, int BLOCK, bool NUM, typename T, typename RP, typename F, int U = 4>
 __device__ inline void for_each_product(const TileDev<T, RP>& tile, RP a0, RP a1, int lane, RP* e_start,
                                         long long* e_pre, T* e_val, long long* scan_smem, const volatile int* stop,
                                         F&& f) {
@@ -332,35 +370,7 @@ __device__ inline void for_each_product(const TileDev<T, RP>& tile, RP a0, RP a1
     if (lane < ne) e_pre[lane] = ex;
     if (lane == 0) e_pre[ne] = total;
     group_sync<G>();
-    for (long long t = lane; t < total; t += 2 * G) {
-      if (stop && *stop) break;
-      const long long t2 = t + G;
-      const bool has2 = t2 < total;
-      int lo = 0, hi = ne;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (e_pre[mid] <= t) lo = mid; else hi = mid;
-      }
-      int lo2 = lo, hi2 = ne;
-      if (has2) {
-        while (hi2 - lo2 > 1) {
-          const int mid = (lo2 + hi2) >> 1;
-          if (e_pre[mid] <= t2) lo2 = mid; else hi2 = mid;
-        }
-      }
-      const RP q1 = e_start[lo] + (RP)(t - e_pre[lo]);
-      const RP q2 = has2 ? e_start[lo2] + (RP)(t2 - e_pre[lo2]) : q1;
-      const int j1 = tile.b_col[q1];
-      const int j2 = tile.b_col[q2];
-      double v1 = 0.0, v2 = 0.0;
-      if (NUM) {
-        const T bv1 = tile.b_val[q1], bv2 = tile.b_val[q2];
-        v1 = (double)e_val[lo] * (double)bv1;
-        v2 = (double)e_val[lo2] * (double)bv2;
-      }
-      f(j1, v1);
-      if (has2) f(j2, v2);
-    }
+    enumerate_staged<G, NUM, U>(tile, ne, total, lane, e_start, e_pre, e_val, stop, f);
     group_sync<G>();
   }
 }
@@ -1033,14 +1043,25 @@ __device__ inline int hr_select(const unsigned* bits, const int* gpre, int ng, i
   return w * 32 + (__ffs(word) - 1);
 }
 
+// First position in B row [st, st + len) whose column is >= c (binary search in global memory).
+template <typename RP>
+__device__ inline RP hr_lower_bound(const int* b_col, RP st, RP len, int c) {
+  RP lo = 0, hi = len;
+  while (lo < hi) {
+    const RP mid = (lo + hi) >> 1;
+    if (b_col[st + mid] < c) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
 template <int MODE, typename T, typename RP>
 __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> R, int bin) {
   constexpr int BLOCK = hr_block<MODE>();
   constexpr bool NUM = MODE != 0;
+  constexpr int kHRSlice = hr_slice<T, RP>();
   __shared__ __align__(16) unsigned s_bits[kHRWords];
   __shared__ int s_gpre[kHRWords / kHRGroup];
-  constexpr int kHRSlice = hr_slice<T, RP>();
-  __shared__ double s_acc[NUM ? kHRSlice : 1];
+  __shared__ __align__(16) double s_acc[NUM ? kHRSlice : 1];  // a slice's columns (as int), then its values
   __shared__ long long s_scan[BLOCK / 64 + 1];
   __shared__ long long s_base;
   __shared__ int s_crange[2];
@@ -1052,6 +1073,7 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
   const int tid = threadIdx.x;
   const int count = R.bin_count[bin];
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
+  int* s_colbuf = reinterpret_cast<int*>(s_acc);
   for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
     const int fr = items[idx];
     const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
@@ -1059,10 +1081,37 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
     const int W = (tile.n + 31) / 32;
     const int Wr = (W + kHRGroup - 1) / kHRGroup * kHRGroup;  // whole groups (the tail words stay zero)
     for (int w = tid; w < Wr; w += BLOCK) s_bits[w] = 0u;
-    __syncthreads();
     const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
-    for_each_product<BLOCK, BLOCK, false, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
-                                                 [&](int j, double) { atomicOr(&s_bits[j >> 5], 1u << (j & 31)); });
+    const int na = (int)(a1 - a0);
+    const bool one_chunk = na <= BLOCK;  // one entry per thread, staged once and kept in registers
+    RP my_st = 0, my_len = 0;
+    T my_av = T(0);
+    if (one_chunk && tid < na) {
+      const int kk = tile.a_col[a0 + tid];
+      my_st = tile.b_rp[kk];
+      my_len = tile.b_rp[kk + 1] - my_st;
+      if (NUM) my_av = tile.a_val[a0 + tid];
+    }
+    // stage (start, length) of every entry from registers and run the product loop
+    auto run_staged = [&](RP st, RP len, auto&& f, auto num_tag) {
+      constexpr bool kNum = decltype(num_tag)::value;
+      if (tid < na) {
+        s_est[tid] = st;
+        if (kNum) s_eval[tid] = my_av;
+      }
+      long long tot;
+      const long long ex = block_excl_scan<BLOCK, long long>((long long)(tid < na ? len : 0), &tot, s_lscan);
+      if (tid < na) s_epre[tid] = ex;
+      if (tid == 0) s_epre[na] = tot;
+      __syncthreads();
+      enumerate_staged<BLOCK, kNum, 4>(tile, na, tot, tid, s_est, s_epre, s_eval, nullptr, f);
+      __syncthreads();
+    };
+    __syncthreads();
+    auto set_bit = [&](int j, double) { atomicOr(&s_bits[j >> 5], 1u << (j & 31)); };
+    if (one_chunk) run_staged(my_st, my_len, set_bit, std::false_type{});
+    else for_each_product<BLOCK, BLOCK, false, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
+                                                      set_bit);
     __syncthreads();
     // group prefixes, in order: thread t counts groups [t * kGroupsPerThread, (t + 1) * kGroupsPerThread)
     const int ng = Wr / kHRGroup;
@@ -1122,32 +1171,49 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
       __syncthreads();
       continue;
     }
-    // columns straight from the bitmap (words dealt round-robin: a row's columns cluster at the R-MAT hubs' ids)
-    for (int w = tid; w < W; w += BLOCK) {
-      unsigned word = s_bits[w];
-      if (!word) continue;
-      long long r = base + hr_rank(s_bits, s_gpre, w * 32);
-      while (word) {
-        tile.c_col[r++] = w * 32 + (__ffs(word) - 1);
-        word &= word - 1;
-      }
-    }
-    // values, one rank slice at a time
+    // one rank slice at a time: columns (staged in LDS by rank, then stored with unit stride), then values
     for (long long s0 = 0; s0 < total; s0 += kHRSlice) {
       const int len = (int)(total - s0 < kHRSlice ? total - s0 : kHRSlice);
-      for (int t = tid; t < len; t += BLOCK) s_acc[t] = 0.0;
-      if (tid == 0) {  // column range of the slice (the whole row when it is one slice)
+      const bool whole = s0 == 0 && len == total;
+      if (tid == 0) {  // column range of the slice
         s_crange[0] = s0 == 0 ? 0 : hr_select(s_bits, s_gpre, ng, (int)s0);
         s_crange[1] = s0 + len >= total ? INT_MAX : hr_select(s_bits, s_gpre, ng, (int)(s0 + len));
       }
       __syncthreads();
       const int c_lo = s_crange[0], c_hi = s_crange[1];
-      for_each_product<BLOCK, BLOCK, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
-                                                  [&](int j, double v) {
-                                                    if (j >= c_lo && j < c_hi)
-                                                      atomicAdd(&s_acc[hr_rank(s_bits, s_gpre, j) - (int)s0], v);
-                                                  });
+      const int w_lo = c_lo >> 5, w_hi = c_hi == INT_MAX ? W - 1 : (c_hi - 1) >> 5;
+      for (int w = w_lo + tid; w <= w_hi; w += BLOCK) {  // words dealt round-robin
+        unsigned word = s_bits[w];
+        if (w == w_lo) word &= ~((1u << (c_lo & 31)) - 1u);
+        if (c_hi != INT_MAX && w == (c_hi >> 5)) word &= (1u << (c_hi & 31)) - 1u;
+        if (!word) continue;
+        int r = hr_rank(s_bits, s_gpre, w * 32 + (__ffs(word) - 1)) - (int)s0;
+        while (word) {
+          s_colbuf[r++] = w * 32 + (__ffs(word) - 1);
+          word &= word - 1;
+        }
+      }
       __syncthreads();
+      for (int t = tid; t < len; t += BLOCK) tile.c_col[base + s0 + t] = s_colbuf[t];
+      __syncthreads();
+      for (int t = tid; t < len; t += BLOCK) s_acc[t] = 0.0;
+      __syncthreads();
+      auto add = [&](int j, double v) {
+        if (whole || (j >= c_lo && j < c_hi)) atomicAdd(&s_acc[hr_rank(s_bits, s_gpre, j) - (int)s0], v);
+      };
+      if (one_chunk) {
+        RP st = my_st, ln = my_len;
+        if (!whole && tid < na) {  // the part of the entry's B row inside the slice's columns
+          const RP lo = s0 == 0 ? 0 : hr_lower_bound(tile.b_col, my_st, my_len, c_lo);
+          const RP hi = c_hi == INT_MAX ? my_len : hr_lower_bound(tile.b_col, my_st, my_len, c_hi);
+          st = my_st + lo;
+          ln = hi - lo;
+        }
+        run_staged(st, ln, add, std::true_type{});
+      } else {
+        for_each_product<BLOCK, BLOCK, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr, add);
+        __syncthreads();
+      }
       for (int t = tid; t < len; t += BLOCK) tile.c_val[base + s0 + t] = (T)s_acc[t];
       __syncthreads();
     }

@@ -48,6 +48,19 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(RowsDev<T, RP> R, con
   }
 }
 
+// Predicted rows -> bins by their predicted capacity (every other row was binned by the analysis pass).
+template <typename T, typename RP>
+__global__ __launch_bounds__(256) void k_bin_predicted(RowsDev<T, RP> R) {
+  const int cnt = *R.pred_count;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cnt; idx += gridDim.x * blockDim.x) {
+    const int fr = R.pred_items[idx];
+    const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
+    const int b = bin_for(R.cap[fr], R.prod[fr], R.tn, tile.n <= (1 << kEscColBits));
+    const int pos = atomicAdd(&R.bin_count[b], 1);
+    R.bin_items[(int64_t)b * R.bin_stride + pos] = fr;
+  }
+}
+
 // Overflow rows of the predicted pass -> LDS bins sized by the exact upper bound min(prod, n).
 template <typename T, typename RP>
 __global__ __launch_bounds__(256) void k_ovf_bin(RowsDev<T, RP> R) {
@@ -195,15 +208,19 @@ struct StagePlan : StageBase {
     join(s);
   }
 
-  // Analysis with capacities (exact bound everywhere), then the network on the listed rows when weights are given.
+  // Analysis with capacities (exact bound everywhere) and binning of every row that is not predicted, then the
+  // network on the listed rows (when weights are given) and their binning by the predicted capacity.
   void predict(hipStream_t s) {
     const bool use_mlp = opts.h_mlp_params != nullptr;
-    F.analyze(s, false, use_mlp ? (long long)kPredictMinProducts : LLONG_MAX);
+    fill_i32(s, F.R.bin_count, kNumBins, 0);
+    fill_i32(s, F.R.pred_count, 1, 0);
+    F.analyze(s, true, use_mlp ? (long long)kPredictMinProducts : LLONG_MAX);
     if (use_mlp) {
       const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(div_up(F.nflat, kPredictBlock),
                                                                             (int64_t)4 * h->num_cus));
       hipLaunchKernelGGL((k_predict<T, RP>), dim3(nb), dim3(kPredictBlock), 0, s, F.R, params.as<float>(),
                          opts.margin, opts.slack);
+      hipLaunchKernelGGL((k_bin_predicted<T, RP>), dim3(std::max<unsigned>(1, h->num_cus)), dim3(256), 0, s, F.R);
       BCLX_HIP(hipGetLastError());
     }
   }
@@ -215,7 +232,6 @@ struct StagePlan : StageBase {
       fill_u64(s, F.R.cursor, 1, (unsigned long long)main_cap);
       predict(s);
       F.scan(s, F.R.cap, F.R.off, 0);
-      F.bin(s, 1);
       run_bins(s, 2, counts1, persistent);
       fill_i32(s, F.R.bin_count, kNumBins, 0);
       hipLaunchKernelGGL((k_ovf_bin<T, RP>), dim3(std::max<unsigned>(1, std::min<unsigned>(div_up(counts_ovf, 256), 1024))),
@@ -286,11 +302,14 @@ struct StagePlan : StageBase {
         }
         F.upload_tiles();
         F.R.out_limit = arena_cap;
+        if (attempt > 0) {  // the previous attempt's overflow redo re-binned: analyse and bin again (same capacities)
+          F.reset_counters(s);
+          predict(s);
+        }
         fill_u64(s, F.R.cursor, 1, (unsigned long long)main_cap);
         fill_u64(s, F.R.stats + 2, 2, 0);
         fill_i32(s, F.R.ovf_count, 1, 0);
         fill_i32(s, F.R.nnz, F.nflat, 0);
-        F.bin(s, 1);
         read_counts(counts1);
         for (int b = 1; b < kNumBins; b++) F.run_bin(s, 2, b, counts1[b]);
         int32_t novf = 0;

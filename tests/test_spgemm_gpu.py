@@ -77,6 +77,16 @@ def test_rmat_heavy_rows(handle):
     check_product(handle, a, a)
 
 
+def test_rmat_sliced_heavy_rows(handle):
+    """R-MAT scale 15 (32768 columns): hub rows have more output entries than one LDS slice of the heavy-row kernel
+    (several slices, each restricted to its column range) and more A entries than one staged chunk."""
+    a = gen.rmat(15, seed=7)
+    nnz, _ = O.row_nnz(a, a)
+    assert (nnz > 9216).sum() > 100, "test needs rows spanning several slices"
+    assert (np.diff(a.row_ptr) > 1024).any(), "test needs rows with more than one chunk of A entries"
+    check_product(handle, a, a)
+
+
 def test_zero_retention(handle):
     """A = [[1, 1]], B = [[1], [-1]]: C keeps the exact-zero entry (spgemm.hpp:69 never erases)."""
     a = CSR(1, 2, np.array([0, 2]), np.array([0, 1], np.int32), np.array([1, 1], np.float32))

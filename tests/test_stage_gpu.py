@@ -104,6 +104,22 @@ def test_stage_tuning_choices(handle, stage14, tuning):
         plan.close()
 
 
+def test_stage_sliced_heavy_rows(handle):
+    """One product of R-MAT scale 15 (32768 columns): heavy rows that span several LDS slices, in the predicted mode
+    (slot or bump allocation) and the one-pass redo of overflowed rows (untrained network)."""
+    a = gen.rmat(15, seed=7)
+    tiles = gen.split_grid(a, 1, 1)
+    A, B, pairs = stage_tiles(tiles, tiles, 1, 0)
+    for prm in (predictor.load(), predictor.init_params(seed=4)):
+        if prm is None:
+            continue
+        plan = StagePlan(handle, A, B, mode=LEARNED, params=prm)
+        plan.execute()
+        torch.cuda.synchronize()
+        check_stage(plan, pairs)
+        plan.close()
+
+
 def test_stage_uneven_grid_learned(handle, stage13):
     A, B, pairs = stage_tiles(stage13, stage13, 4, 1)
     for mode in (BATCHED_EXACT, LEARNED):
