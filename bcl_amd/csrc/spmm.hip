@@ -6,10 +6,10 @@
 // Work split: A's nonzeros are cut into chunks of kChunk consecutive entries; one wavefront takes a chunk, finds the
 // row holding its first entry by binary search in row_ptr, and walks the chunk's entries in order. Each lane owns
 // VEC consecutive columns of a 64*VEC column pass; for every entry the wavefront gathers one B row segment
-// (coalesced), multiplies by the entry's value (broadcast from the lane that loaded it) and accumulates in fp32.
-// A row that lies wholly inside the chunk is written with one read-modify-write of its C segment; a row cut by a
-// chunk boundary is added with fp32 atomics (each chunk adds its own part). B is fp32 or bf16 (converted on load);
-// accumulation is fp32.
+// (coalesced; the loads of eight entries are issued together), multiplies by the entry's value (broadcast from the
+// lane that loaded it) and accumulates in fp32.
+// Each finished row (or a chunk's part of a row cut by a chunk boundary) is added to C with fp32 atomics that return
+// nothing, so no wavefront waits on a read of C. B is fp32 or bf16 (converted on load); accumulation is fp32.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -18,6 +18,7 @@ namespace bclx {
 
 constexpr int kSpmmChunk = 256;  // entries per wavefront
 constexpr int kSpmmWaves = 4;    // wavefronts per block
+constexpr int kSpmmUnroll = 8;   // B-row loads in flight per wavefront
 
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -47,6 +48,15 @@ struct BLoad<BT, 1> {
   }
 };
 
+// C segment += alpha * acc with fp32 atomics that return nothing: the wavefront never waits on a read of C, and a row
+// split between chunks sums its parts in L2.
+template <int VEC>
+__device__ inline void spmm_flush(float* cr, const float* acc, float alpha, int left) {
+#pragma unroll
+  for (int v = 0; v < VEC; v++)
+    if (v < left) __hip_atomic_fetch_add(&cr[v], alpha * acc[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename RP, typename BT, int VEC>
 __global__ __launch_bounds__(64 * kSpmmWaves) void k_spmm(const RP* __restrict__ rp, const int* __restrict__ col,
                                                           const float* __restrict__ val, int64_t m, int64_t nnz,
@@ -64,11 +74,13 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void k_spmm(const RP* __restrict__
     if ((int64_t)rp[mid + 1] <= e0) lo = mid + 1;
     else hi = mid;
   }
+  // row ends of the chunk's first 64 rows, one per lane (rows further on are read when reached)
+  const int64_t my_re = lo + 1 + lane <= m ? (int64_t)rp[lo + 1 + lane] : nnz;
   for (int c0 = 0; c0 < n; c0 += 64 * VEC) {
     const int cj = c0 + lane * VEC;
     const bool cok = cj < n;
     int64_t r = lo;
-    int64_t rs = (int64_t)rp[r], re = (int64_t)rp[r + 1];
+    int64_t rs = (int64_t)rp[r], re = __shfl(my_re, 0, 64);
     float acc[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; v++) acc[v] = 0.f;
@@ -78,52 +90,43 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void k_spmm(const RP* __restrict__
       const int my_col = eq < e1 ? col[eq] : 0;
       const float my_val = eq < e1 ? val[eq] : 0.f;
       const int cnt = (int)((e1 - eb) < 64 ? (e1 - eb) : 64);
-      for (int q = 0; q < cnt; q++) {
-        const int64_t e = eb + q;
-        while (e >= re) {  // flush the finished row (empty rows have nothing to add), move to the row holding e
-          if (cok && re > rs) {
-            float* cr = C + r * ldc + cj;
-            const bool whole = rs >= e0;  // row started inside this chunk and ends here
+      for (int q0 = 0; q0 < cnt; q0 += kSpmmUnroll) {
+        // issue the B-row loads of kSpmmUnroll entries before using any, so their latencies overlap
+        float b[kSpmmUnroll][VEC];
+        float a[kSpmmUnroll];
 #pragma unroll
-            for (int v = 0; v < VEC; v++) {
-              if (cj + v >= n) break;
-              if (whole) cr[v] += alpha * acc[v];
-              else atomicAdd(&cr[v], alpha * acc[v]);
+        for (int u = 0; u < kSpmmUnroll; u++) {
+          const int q = q0 + u < cnt ? q0 + u : cnt - 1;
+          const int k = __shfl(my_col, q, 64);
+          a[u] = __shfl(my_val, q, 64);
+#pragma unroll
+          for (int v = 0; v < VEC; v++) b[u][v] = 0.f;
+          if (cok && q0 + u < cnt) {
+            if (cj + VEC <= n) {
+              BLoad<BT, VEC>::load(B + (int64_t)k * ldb + cj, b[u]);
+            } else {
+              for (int v = 0; v < VEC && cj + v < n; v++) BLoad<BT, 1>::load(B + (int64_t)k * ldb + cj + v, &b[u][v]);
             }
           }
-#pragma unroll
-          for (int v = 0; v < VEC; v++) acc[v] = 0.f;
-          r++;
-          rs = re;
-          re = (int64_t)rp[r + 1];
         }
-        const int k = __shfl(my_col, q, 64);
-        const float a = __shfl(my_val, q, 64);
-        if (cok) {
-          float b[VEC];
-          if (cj + VEC <= n) {
-            BLoad<BT, VEC>::load(B + (int64_t)k * ldb + cj, b);
-          } else {
 #pragma unroll
-            for (int v = 0; v < VEC; v++) b[v] = 0.f;
-            for (int v = 0; v < VEC && cj + v < n; v++) BLoad<BT, 1>::load(B + (int64_t)k * ldb + cj + v, &b[v]);
+        for (int u = 0; u < kSpmmUnroll; u++) {
+          if (q0 + u >= cnt) break;
+          const int64_t e = eb + q0 + u;
+          while (e >= re) {  // flush the finished row (empty rows have nothing to add), move to the row holding e
+            if (cok && re > rs) spmm_flush<VEC>(C + r * ldc + cj, acc, alpha, n - cj);
+#pragma unroll
+            for (int v = 0; v < VEC; v++) acc[v] = 0.f;
+            r++;
+            rs = re;
+            re = r - lo < 64 ? __shfl(my_re, (int)(r - lo), 64) : (int64_t)rp[r + 1];
           }
 #pragma unroll
-          for (int v = 0; v < VEC; v++) acc[v] = fmaf(a, b[v], acc[v]);
+          for (int v = 0; v < VEC; v++) acc[v] = fmaf(a[u], b[u][v], acc[v]);
         }
       }
     }
-    // last row of the chunk: whole only if it started inside the chunk and ends at e1
-    if (cok) {
-      float* cr = C + r * ldc + cj;
-      const bool whole = rs >= e0 && re <= e1;
-#pragma unroll
-      for (int v = 0; v < VEC; v++) {
-        if (cj + v >= n) break;
-        if (whole) cr[v] += alpha * acc[v];
-        else atomicAdd(&cr[v], alpha * acc[v]);
-      }
-    }
+    if (cok) spmm_flush<VEC>(C + r * ldc + cj, acc, alpha, n - cj);  // the chunk's last (possibly partial) row
   }
 }
 

@@ -32,6 +32,19 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) for this
+    workload, from the committed summary profiles/<name>_hbm_traffic.md; None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{name}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
+
+
 def csr_bytes(m, nnz, w_ptr=4, w_val=4):
     return (m + 1) * w_ptr + nnz * (4 + w_val)
 
@@ -181,7 +194,8 @@ def bench_stage(args):
         "hbm_gbs": round(gbs, 2),
         "hbm_pct_peak": round(100 * gbs / HBM_PEAK_GBS, 3),
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 5),
+                     "traffic": profiled_traffic(f"stage_rmat_s{args.scale}_grid{args.grid}_k{args.stage_k}"),
                      "kernel": "the stage's single graph launch (all pipeline kernels)",
                      "algorithmic_bytes": int(alg_bytes)},
         "learned": {"overflow_rows": st["overflow_rows"], "predicted_rows": st["predicted_rows"],
@@ -296,7 +310,8 @@ def bench_spmm(args):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (B fp32 or bf16)",
             "data": "synthetic", "config": {"workload": f"rmat-s{args.spmm_scale}-ef16-x-dense128", "nnz_a": a.nnz},
             "roofline": {"bound": "hbm", "achieved": f["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(f["hbm_gbs"] / HBM_PEAK_GBS, 5), "traffic": None,
+                         "frac": round(f["hbm_gbs"] / HBM_PEAK_GBS, 5),
+                         "traffic": profiled_traffic(f"spmm_rmat_s{args.spmm_scale}_n128"),
                          "algorithmic_bytes": f["algorithmic_bytes"], "kernel": "k_spmm (fp32 B)"},
             "fp32_b": out["fp32"], "bf16_b": out["bf16"],
             "cpu_baseline": {"value": round(2 * sub2.nnz * n / dt / 1e9, 4), "unit": "GFLOP/s", "cores": 1,
