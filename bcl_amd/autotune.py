@@ -22,11 +22,11 @@ import time
 import numpy as np
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
-DEFAULT = {"work_per_lane": 64, "heavy_need": 4096, "load_shift": 1, "esc_min": 32, "esc_max": 4096,
+DEFAULT = {"work_per_lane": 64, "heavy_need": 4096, "load_shift": 1, "esc_min": 32, "esc_max": 4096, "bin_streams": 1,
            "margin": 0.15, "slack": 4}
-SPACE = {"esc_max": [0, 512, 1024, 2048, 4096], "esc_min": [0, 16, 32, 64], "work_per_lane": [16, 32, 64, 128, 256],
+SPACE = {"bin_streams": [1, 2, 4], "esc_max": [0, 512, 1024, 2048, 4096], "esc_min": [0, 16, 32, 64], "work_per_lane": [16, 32, 64, 128, 256],
          "heavy_need": [1024, 2048, 4096], "load_shift": [1, 2], "margin": [0.05, 0.1, 0.15, 0.25, 0.4]}
-KNOBS = ("work_per_lane", "heavy_need", "load_shift", "esc_min", "esc_max")
+KNOBS = ("work_per_lane", "heavy_need", "load_shift", "esc_min", "esc_max", "bin_streams")
 
 
 def time_plan(h, A, B, params, cfg, reps=3):

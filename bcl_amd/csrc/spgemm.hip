@@ -85,6 +85,7 @@ void FlatRows<T, RP>::setup(bclx_handle* h_, const std::vector<TileDev<T, RP>>& 
   size_t o_oi = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
   size_t o_pi = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
   size_t o_cur = A.reserve(sizeof(unsigned long long) * 2);
+  size_t o_tmp = A.reserve(sizeof(int32_t) * nflat);
   size_t o_stats = A.reserve(sizeof(unsigned long long) * 8);
   size_t nsb = (size_t)((nflat + kScanTile - 1) / kScanTile) + 1;
   size_t o_part = A.reserve(sizeof(long long) * nsb);
@@ -104,6 +105,8 @@ void FlatRows<T, RP>::setup(bclx_handle* h_, const std::vector<TileDev<T, RP>>& 
   R.pred_count = R.ovf_count + 1;
   R.pred_items = A.at<int32_t>(o_pi);
   R.cursor = A.at<unsigned long long>(o_cur);
+  R.heavy_queue = R.cursor + 1;
+  R.tmp_items = A.at<int32_t>(o_tmp);
   R.stats = A.at<unsigned long long>(o_stats);
   scan_partial = A.at<long long>(o_part);
   scan_total = A.at<int64_t>(o_tot);
@@ -249,6 +252,8 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
   if (b == kHeavyBin && (max_n + 31) / 32 <= kHRWords) {
     const int64_t resident = (mode == 0 ? 2 : 1) * (int64_t)h->num_cus;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(count_hint >= 0 ? count_hint : resident, resident));
+    fill_u64(s, R.heavy_queue, 1, 0ull);
+    hipLaunchKernelGGL((k_order_heavy<T, RP>), dim3(1), dim3(1024), 0, s, R);
     switch (mode) {
       case 0: hipLaunchKernelGGL((k_heavy_rank<0, T, RP>), dim3(grid), dim3(hr_block<0>()), 0, s, R, kHeavyBin); break;
       case 1: hipLaunchKernelGGL((k_heavy_rank<1, T, RP>), dim3(grid), dim3(hr_block<1>()), 0, s, R, kHeavyBin); break;

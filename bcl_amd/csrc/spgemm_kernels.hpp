@@ -103,6 +103,8 @@ struct RowsDev {
   int32_t* pred_count;       // rows whose capacity the predictor sets (learned path)
   int32_t* pred_items;
   unsigned long long* cursor;  // bump allocator for rows placed outside their predicted slot
+  unsigned long long* heavy_queue;  // next heavy row to take (rows are handed out largest first)
+  int32_t* tmp_items;               // scratch list for ordering the heavy bin
   unsigned long long* stats;   // [0] products [1] heavy rows [2] overflow rows [3] capacity exceeded [4] predicted
   long long out_limit;         // entries available in the output arrays; writes beyond it are refused
   BinTuning tn;
@@ -1054,6 +1056,38 @@ __device__ inline RP hr_lower_bound(const int* b_col, RP st, RP len, int c) {
   return lo;
 }
 
+// Orders the heavy bin by decreasing products (classes of equal floor(log2 P)), so that the dynamic queue of
+// k_heavy_rank starts the largest rows first and the small ones fill in behind them. One block.
+template <typename T, typename RP>
+__global__ __launch_bounds__(1024) void k_order_heavy(RowsDev<T, RP> R) {
+  __shared__ int s_hist[32], s_next[32];
+  const int cnt = R.bin_count[kHeavyBin];
+  int* items = R.bin_items + (int64_t)kHeavyBin * R.bin_stride;
+  if (threadIdx.x < 32) s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  auto cls = [&](int fr) {  // 31 - floor(log2 P), clamped to [0, 31]: larger rows first
+    const long long p = R.prod[fr];
+    const int lg = 63 - __clzll((unsigned long long)(p > 1 ? p : 1));
+    return 31 - (lg < 31 ? lg : 31);
+  };
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) atomicAdd(&s_hist[cls(items[q])], 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int c = 0; c < 32; c++) {
+      s_next[c] = run;
+      run += s_hist[c];
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    const int fr = items[q];
+    R.tmp_items[atomicAdd(&s_next[cls(fr)], 1)] = fr;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) items[q] = R.tmp_items[q];
+}
+
 template <int MODE, typename T, typename RP>
 __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> R, int bin) {
   constexpr int BLOCK = hr_block<MODE>();
@@ -1074,7 +1108,14 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
   const int count = R.bin_count[bin];
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
   int* s_colbuf = reinterpret_cast<int*>(s_acc);
-  for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
+  __shared__ int s_idx;
+  for (;;) {
+    // rows are taken from a queue (largest first), so one block's share is not fixed in advance
+    if (tid == 0) s_idx = (int)atomicAdd(R.heavy_queue, 1ull);
+    __syncthreads();
+    const int idx = s_idx;
+    __syncthreads();
+    if (idx >= count) break;
     const int fr = items[idx];
     const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
     const int64_t i = fr - tile.flat0;

@@ -103,6 +103,7 @@ struct StagePlan : StageBase {
   int mode, npairs;
   bclx_stage_opts opts{};
   BinTuning tn;
+  int bin_streams = 1;  // graph branches the bins run on (1: one after another)
   // mode 0
   std::vector<std::unique_ptr<TwoPhase<T, RP>>> per;  // one two-phase workspace per product
   std::vector<DevBuf> c_rp, c_col, c_val;
@@ -129,13 +130,14 @@ struct StagePlan : StageBase {
       tn.load_shift = std::min(std::max(1, opts.h_tuning[2]), 2);
       tn.esc_min = std::max(0, opts.h_tuning[3]);
       tn.esc_max = std::min(std::max(0, opts.h_tuning[4]), kEscMaxProducts);
+      bin_streams = std::min(std::max(1, opts.h_tuning[5]), 8);
     }
     std::vector<TileDev<T, RP>> tiles;
     for (int p = 0; p < np; p++) tiles.push_back(make_tile<T, RP>(&A[p], &B[p]));
     if (mode == 0) {
       setup_plain(tiles);
     } else {
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < bin_streams; q++) {
         hipStream_t s;
         BCLX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         side.push_back(s);

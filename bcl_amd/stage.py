@@ -15,8 +15,8 @@ from .csr import CSR
 from .device import DeviceCSR, Handle
 
 PLAIN, LEARNED, BATCHED_EXACT = 0, 1, 2
-# BinTuning defaults (bclx_stage_opts.h_tuning): work_per_lane, heavy_need, load_shift, esc_min, esc_max
-TUNING_DEFAULTS = (64, 4096, 1, 32, 4096)
+# bclx_stage_opts.h_tuning defaults: work_per_lane, heavy_need, load_shift, esc_min, esc_max, bin_streams
+TUNING_DEFAULTS = (64, 4096, 1, 32, 4096, 1)
 
 
 def full_tuning(t) -> np.ndarray:

@@ -62,7 +62,7 @@ typedef struct bclx_handle bclx_handle;
 typedef struct bclx_spgemm_plan bclx_spgemm_plan;
 typedef struct bclx_stage_plan bclx_stage_plan;
 
-#define BCLX_TUNING_LEN 5
+#define BCLX_TUNING_LEN 6
 
 /* Last error message of the calling thread (empty string when none). */
 BCLX_API const char* bclx_last_error(void);
@@ -107,7 +107,8 @@ typedef struct {
                               *   [1] heavy_need (4096)   rows needing more slots use the heavy accumulator
                               *   [2] load_shift (1)      hash slots >= need << load_shift
                               *   [3] esc_min (32)        rows with esc_min < products <= esc_max are sorted
-                              *   [4] esc_max (4096)      instead of hashed (0 turns sorting off; max 4096) */
+                              *   [4] esc_max (4096)      instead of hashed (0 turns sorting off; max 4096)
+                              *   [5] bin_streams (1)     concurrent graph branches for the bins (1..8) */
 } bclx_stage_opts;
 
 BCLX_API bclx_status bclx_stage_plan_create(bclx_handle* h, int npairs, const bclx_csr* A, const bclx_csr* B,
