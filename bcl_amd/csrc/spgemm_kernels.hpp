@@ -654,6 +654,25 @@ __device__ inline bool esc_head(const unsigned (&key)[K], int k, int lane) {
   return key[k] != kEscPad && ((k == 0 && lane == 0) || (prev >> kEscKeyShift) != (key[k] >> kEscKeyShift));
 }
 
+// For the K products x = x0 + k * 64 + lane of a wavefront, the staged entry e holding product t = x - base
+// (pre[e] <= t < pre[e + 1], pre has ne + 1 prefix counts): K branchless binary searches in lockstep. Products outside
+// the chunk get some valid e (the caller skips them).
+template <int K>
+__device__ inline void esc_lookup(const int* pre, int ne, int base, int lane, int x0, int (&lo)[K]) {
+  int top = 1;  // largest power of two < ne (0 when ne == 1)
+  while (top < ne) top <<= 1;
+  top >>= 1;
+#pragma unroll
+  for (int k = 0; k < K; k++) lo[k] = 0;
+  for (int step = top; step > 0; step >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int c = lo[k] + step;
+      if (c < ne && pre[c] <= x0 + k * 64 + lane - base) lo[k] = c;
+    }
+  }
+}
+
 // Compare-exchange of one bitonic stage whose partner is `d` lanes away (d < 64): keep min or max of the pair.
 template <int D>
 __device__ inline unsigned esc_xchg(unsigned v, bool keep_min) {
@@ -744,19 +763,16 @@ __global__ __launch_bounds__(64 * kEscWaves) void k_esc(RowsDev<T, RP> R, int bi
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      int lo[K];  // entry e with s_pre[e] <= t < s_pre[e + 1], all K lookups in lockstep
+      esc_lookup<K>(s_pre[w], ne, base, lane, 0, lo);
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int x = k * 64 + lane;
         const int t = x - base;
         if (t >= 0 && t < total) {
-          int lo = 0, hi = ne;  // entry e with s_pre[e] <= t < s_pre[e + 1]
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_pre[w][mid] <= t) lo = mid; else hi = mid;
-          }
-          const RP q = s_st[w][lo] + (RP)(t - s_pre[w][lo]);
+          const RP q = s_st[w][lo[k]] + (RP)(t - s_pre[w][lo[k]]);
           key[k] = ((unsigned)tile.b_col[q] << kEscKeyShift) | (unsigned)x;
-          if (NUM) s_val[w][x] = (double)s_av[w][lo] * (double)tile.b_val[q];
+          if (NUM) s_val[w][x] = (double)s_av[w][lo[k]] * (double)tile.b_val[q];
         }
       }
       base += total;
@@ -875,19 +891,16 @@ __global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int b
       }
       __syncthreads();
       const int total = s_pre[ne];
+      int lo[K];
+      esc_lookup<K>(s_pre, ne, base, lane, w * kEscChunk, lo);
 #pragma unroll
       for (int k = 0; k < K; k++) {
         const int x = w * kEscChunk + k * 64 + lane;
         const int t = x - base;
         if (t >= 0 && t < total) {
-          int lo = 0, hi = ne;
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_pre[mid] <= t) lo = mid; else hi = mid;
-          }
-          const RP q = s_st[lo] + (RP)(t - s_pre[lo]);
+          const RP q = s_st[lo[k]] + (RP)(t - s_pre[lo[k]]);
           key[k] = ((unsigned)tile.b_col[q] << kEscKeyShift) | (unsigned)x;
-          if (NUM) s_val[x] = (double)s_av[lo] * (double)tile.b_val[q];
+          if (NUM) s_val[x] = (double)s_av[lo[k]] * (double)tile.b_val[q];
         }
       }
       base += total;
@@ -900,24 +913,31 @@ __global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int b
     __syncthreads();
 #pragma unroll
     for (int L = kEscChunk; L < N; L <<= 1) {
-      int dst[K];
+      // a wavefront's 512 positions lie in one run (L is a multiple of 512), so its partner run is uniform
+      const int run = (w * kEscChunk) / L;
+      const int other = (run ^ 1) * L;
+      const bool left = (run & 1) == 0;
+      const int base_out = (run & ~1) * L + (w * kEscChunk - run * L) + lane;  // + k * 64 + rank
+      // rank of each key in the partner run (left run: partner keys < key; right run: partner keys <= key), all K
+      // binary searches in lockstep with power-of-two steps so their LDS latencies overlap
+      int rank[K];
+#pragma unroll
+      for (int k = 0; k < K; k++) rank[k] = 0;
+      for (int step = L >> 1; step > 0; step >>= 1) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          const unsigned o = s_key[other + rank[k] + step - 1];
+          if (o < key[k] || (!left && o == key[k])) rank[k] += step;
+        }
+      }
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const int p = w * kEscChunk + k * 64 + lane;
-        const int run = p / L, pos = p - run * L;
-        const int other = (run ^ 1) * L;
-        const bool left = (run & 1) == 0;
-        int lo = 0, hi = L;  // left run: partner keys < key; right run: partner keys <= key
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          const unsigned o = s_key[other + mid];
-          if (o < key[k] || (!left && o == key[k])) lo = mid + 1; else hi = mid;
-        }
-        dst[k] = (run & ~1) * L + pos + lo;
+        const unsigned o = s_key[other + rank[k]];
+        if (o < key[k] || (!left && o == key[k])) rank[k] += 1;
       }
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < K; k++) s_key[dst[k]] = key[k];
+      for (int k = 0; k < K; k++) s_key[base_out + k * 64 + rank[k]] = key[k];
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < K; k++) key[k] = s_key[w * kEscChunk + k * 64 + lane];
