@@ -674,9 +674,26 @@ __device__ inline void esc_lookup(const int* pre, int ne, int base, int lane, in
 }
 
 // Compare-exchange of one bitonic stage whose partner is `d` lanes away (d < 64): keep min or max of the pair.
+// Value of lane (lane ^ D) of a wavefront. Distances 1, 2 and 8 are one DPP move (quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_ror:8), 4 is two DPP moves (row_shl:4 for lanes with bit 2 clear, row_shr:4 for the others), 16 is
+// a ds_swizzle in xor mode (within 32 lanes), 32 a bpermute.
 template <int D>
-__device__ inline unsigned esc_xchg(unsigned v, bool keep_min) {
-  const unsigned o = (unsigned)__shfl_xor((int)v, D, 64);
+__device__ inline unsigned lane_xor(unsigned v, int lane) {
+  const int x = (int)v;
+  if constexpr (D == 1) return (unsigned)__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+  else if constexpr (D == 2) return (unsigned)__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+  else if constexpr (D == 4) {
+    const int up = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0xF, false);  // lane + 4
+    const int dn = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // lane - 4
+    return (unsigned)((lane & 4) ? dn : up);
+  } else if constexpr (D == 8) return (unsigned)__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);
+  else if constexpr (D == 16) return (unsigned)__builtin_amdgcn_ds_swizzle(x, 0x401F);
+  else return (unsigned)__shfl_xor(x, D, 64);
+}
+
+template <int D>
+__device__ inline unsigned esc_xchg(unsigned v, bool keep_min, int lane) {
+  const unsigned o = lane_xor<D>(v, lane);
   return keep_min ? min(v, o) : max(v, o);
 }
 
@@ -706,12 +723,12 @@ __device__ inline void esc_sort(unsigned (&key)[K], int lane) {
           const bool lower = (lane & d) == 0;
           const bool keep_min = asc == lower;
           switch (d) {
-            case 1: key[k] = esc_xchg<1>(key[k], keep_min); break;
-            case 2: key[k] = esc_xchg<2>(key[k], keep_min); break;
-            case 4: key[k] = esc_xchg<4>(key[k], keep_min); break;
-            case 8: key[k] = esc_xchg<8>(key[k], keep_min); break;
-            case 16: key[k] = esc_xchg<16>(key[k], keep_min); break;
-            default: key[k] = esc_xchg<32>(key[k], keep_min); break;
+            case 1: key[k] = esc_xchg<1>(key[k], keep_min, lane); break;
+            case 2: key[k] = esc_xchg<2>(key[k], keep_min, lane); break;
+            case 4: key[k] = esc_xchg<4>(key[k], keep_min, lane); break;
+            case 8: key[k] = esc_xchg<8>(key[k], keep_min, lane); break;
+            case 16: key[k] = esc_xchg<16>(key[k], keep_min, lane); break;
+            default: key[k] = esc_xchg<32>(key[k], keep_min, lane); break;
           }
         }
       }
