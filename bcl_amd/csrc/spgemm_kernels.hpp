@@ -1260,15 +1260,28 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
       __syncthreads();
       const int c_lo = s_crange[0], c_hi = s_crange[1];
       const int w_lo = c_lo >> 5, w_hi = c_hi == INT_MAX ? W - 1 : (c_hi - 1) >> 5;
-      for (int w = w_lo + tid; w <= w_hi; w += BLOCK) {  // words dealt round-robin
-        unsigned word = s_bits[w];
-        if (w == w_lo) word &= ~((1u << (c_lo & 31)) - 1u);
-        if (c_hi != INT_MAX && w == (c_hi >> 5)) word &= (1u << (c_hi & 31)) - 1u;
-        if (!word) continue;
-        int r = hr_rank(s_bits, s_gpre, w * 32 + (__ffs(word) - 1)) - (int)s0;
-        while (word) {
-          s_colbuf[r++] = w * 32 + (__ffs(word) - 1);
-          word &= word - 1;
+      // a wavefront takes 64 consecutive words (four 16-word groups) at a time: each word's rank base is its group
+      // prefix plus the bits of the group's earlier words (a 16-lane scan of popcounts)
+      for (int wb = (w_lo & ~63) + (tid & ~63); wb <= w_hi; wb += BLOCK) {
+        const int lane = tid & 63, w = wb + lane;
+        const unsigned full = w < W ? s_bits[w] : 0u;
+        const int pc = __popc(full);
+        int inc = pc;
+#pragma unroll
+        for (int d = 1; d < kHRGroup; d <<= 1) {
+          const int t = __shfl_up(inc, d, kHRGroup);
+          if ((lane & (kHRGroup - 1)) >= d) inc += t;
+        }
+        unsigned m = (w >= w_lo && w <= w_hi) ? full : 0u;
+        if (w == w_lo) m &= ~((1u << (c_lo & 31)) - 1u);
+        if (c_hi != INT_MAX && w == (c_hi >> 5)) m &= (1u << (c_hi & 31)) - 1u;
+        if (m) {
+          const int r0 = s_gpre[w / kHRGroup] + inc - pc - (int)s0;
+          while (m) {
+            const int b = __ffs(m) - 1;
+            s_colbuf[r0 + __popc(full & ((1u << b) - 1u))] = w * 32 + b;
+            m &= m - 1;
+          }
         }
       }
       __syncthreads();

@@ -51,13 +51,26 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(RowsDev<T, RP> R, con
 // Predicted rows -> bins by their predicted capacity (every other row was binned by the analysis pass).
 template <typename T, typename RP>
 __global__ __launch_bounds__(256) void k_bin_predicted(RowsDev<T, RP> R) {
+  // per block: count in LDS, one global atomic per bin, then place (the rows of a block share few bins)
+  __shared__ int s_cnt[kNumBins], s_base[kNumBins];
   const int cnt = *R.pred_count;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cnt; idx += gridDim.x * blockDim.x) {
-    const int fr = R.pred_items[idx];
-    const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
-    const int b = bin_for(R.cap[fr], R.prod[fr], R.tn, tile.n <= (1 << kEscColBits));
-    const int pos = atomicAdd(&R.bin_count[b], 1);
-    R.bin_items[(int64_t)b * R.bin_stride + pos] = fr;
+  for (int i0 = blockIdx.x * blockDim.x; i0 < cnt; i0 += gridDim.x * blockDim.x) {
+    if (threadIdx.x < kNumBins) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int idx = i0 + threadIdx.x;
+    int fr = 0, b = 0, pos = 0;
+    if (idx < cnt) {
+      fr = R.pred_items[idx];
+      const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
+      b = bin_for(R.cap[fr], R.prod[fr], R.tn, tile.n <= (1 << kEscColBits));
+      pos = atomicAdd(&s_cnt[b], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kNumBins && s_cnt[threadIdx.x] > 0)
+      s_base[threadIdx.x] = atomicAdd(&R.bin_count[threadIdx.x], s_cnt[threadIdx.x]);
+    __syncthreads();
+    if (idx < cnt) R.bin_items[(int64_t)b * R.bin_stride + s_base[b] + pos] = fr;
+    __syncthreads();
   }
 }
 
