@@ -190,9 +190,14 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_
   if (valid) {
     RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
     if (a1 - a0 <= 32) {
-      for (RP p = a0; p < a1; p++) {
-        int k = tile.a_col[p];
-        prod += (long long)(tile.b_rp[k + 1] - tile.b_rp[k]);
+      // eight entries at a time: their column loads, then their B-row-pointer loads, are issued together
+      for (RP p0 = a0; p0 < a1; p0 += 8) {
+        int k[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) k[u] = p0 + u < a1 ? tile.a_col[p0 + u] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (k[u] >= 0) prod += (long long)(tile.b_rp[k[u] + 1] - tile.b_rp[k[u]]);
       }
     } else {
       s_long[atomicAdd(&s_nlong, 1)] = tid;
@@ -1525,16 +1530,20 @@ __global__ __launch_bounds__(kRowBlock) void k_compact(RowsDev<T, RP> R, const i
     for (long long x0 = e; x0 < we; x0 += 4LL * kRowBlock) {
       long long src[4];
       bool ok[4];
+      int a[4];  // largest a with s_dst[a] <= x: four branchless searches in lockstep
+#pragma unroll
+      for (int q = 0; q < 4; q++) a[q] = 0;
+#pragma unroll
+      for (int step = kRowBlock / 2; step > 0; step >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (s_dst[a[q] + step] <= x0 + (long long)q * kRowBlock + tid) a[q] += step;
+      }
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const long long x = x0 + (long long)q * kRowBlock + tid;
         ok[q] = x < we;
-        int a = 0, b = kRowBlock;  // largest a with s_dst[a] <= x
-        while (b - a > 1) {
-          const int mid = (a + b) >> 1;
-          if (s_dst[mid] <= x) a = mid; else b = mid;
-        }
-        src[q] = s_src[a] + (x - s_dst[a]);
+        src[q] = s_src[a[q]] + (x - s_dst[a[q]]);
       }
       int cv[4];
       T vv[4];
