@@ -34,7 +34,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kGBM = 128, kGBN = 128, kGThreads = 256;
+constexpr int kGThreads = 256;
 
 __device__ inline uint16_t f32_to_bf16_rne(float x) {
   uint32_t u = __float_as_uint(x);
@@ -95,8 +95,11 @@ struct GemmCfg<2> {
   static constexpr int BK = 64, PAD = 8, KSTEP = 16;
 };
 
-template <int MODE>
-__global__ __launch_bounds__(kGThreads) void k_gemm(const typename GemmCfg<MODE>::E* __restrict__ Ahi,
+// Block tile (64 * TM) x (64 * TN): four wavefronts in a 2 x 2 arrangement, each owning TM x TN MFMA tiles of
+// 32 x 32. Larger wavefront tiles reuse each LDS fragment for more MFMAs (TM + TN fragment loads per k step feed
+// TM * TN MFMAs).
+template <int MODE, int TM, int TN>
+__global__ __launch_bounds__(kGThreads, TM * TN > 4 ? 2 : 1) void k_gemm(const typename GemmCfg<MODE>::E* __restrict__ Ahi,
                                                     const typename GemmCfg<MODE>::E* __restrict__ Alo,
                                                     const typename GemmCfg<MODE>::E* __restrict__ Bhi,
                                                     const typename GemmCfg<MODE>::E* __restrict__ Blo, int64_t M,
@@ -104,19 +107,21 @@ __global__ __launch_bounds__(kGThreads) void k_gemm(const typename GemmCfg<MODE>
                                                     int64_t ldc, int tiles_m, int tiles_n) {
   using E = typename GemmCfg<MODE>::E;
   constexpr int BK = GemmCfg<MODE>::BK, PAD = GemmCfg<MODE>::PAD, KSTEP = GemmCfg<MODE>::KSTEP;
+  constexpr int BM = 64 * TM, BN = 64 * TN;
   constexpr int LDK = BK + PAD;
   constexpr int NSPLIT = MODE == 2 ? 2 : 1;
-  constexpr int VPR = BK * (int)sizeof(E) / 16;      // 16-byte vectors per tile row
-  constexpr int VPT = kGBM * VPR / kGThreads;        // vectors per thread per operand
-  __shared__ __align__(16) E As[NSPLIT][kGBM * LDK];
-  __shared__ __align__(16) E Bs[NSPLIT][kGBN * LDK];
+  constexpr int VPR = BK * (int)sizeof(E) / 16;   // 16-byte vectors per tile row
+  constexpr int VPA = BM * VPR / kGThreads;       // vectors per thread, A tile
+  constexpr int VPB = BN * VPR / kGThreads;       // vectors per thread, B tile
+  __shared__ __align__(16) E As[NSPLIT][BM * LDK];
+  __shared__ __align__(16) E Bs[NSPLIT][BN * LDK];
 
   // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous run of tiles (row-major over C tiles)
   const int T = tiles_m * tiles_n;
   int b = blockIdx.x;
   if (T % 8 == 0) b = (b % 8) * (T / 8) + (b / 8);
   const int tm = b / tiles_n, tn = b % tiles_n;
-  const int64_t m0 = (int64_t)tm * kGBM, n0 = (int64_t)tn * kGBN;
+  const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -127,39 +132,45 @@ __global__ __launch_bounds__(kGThreads) void k_gemm(const typename GemmCfg<MODE>
   const E* __restrict__ Bg1 = MODE == 2 ? Blo + n0 * Kp : Bg0;
   constexpr int EPV = 16 / (int)sizeof(E);  // elements per 16-byte vector
   // per-thread vector slots: row = (tid + i * 256) / VPR, column vector = (tid + i * 256) % VPR
-  u32x4 ra0[VPT], rb0[VPT], ra1[VPT], rb1[VPT];
+  u32x4 ra0[VPA], rb0[VPB], ra1[VPA], rb1[VPB];
   auto gload = [&](int64_t k0) {
-    static_for<0, VPT>([&](auto ic) {
+    static_for<0, VPA>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const int v = tid + i * kGThreads;
       const int64_t o = (int64_t)(v / VPR) * Kp + k0 + (v % VPR) * EPV;
       ra0[i] = *reinterpret_cast<const u32x4*>(Ag0 + o);
+      if constexpr (MODE == 2) ra1[i] = *reinterpret_cast<const u32x4*>(Ag1 + o);
+    });
+    static_for<0, VPB>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int v = tid + i * kGThreads;
+      const int64_t o = (int64_t)(v / VPR) * Kp + k0 + (v % VPR) * EPV;
       rb0[i] = *reinterpret_cast<const u32x4*>(Bg0 + o);
-      if constexpr (MODE == 2) {
-        ra1[i] = *reinterpret_cast<const u32x4*>(Ag1 + o);
-        rb1[i] = *reinterpret_cast<const u32x4*>(Bg1 + o);
-      }
+      if constexpr (MODE == 2) rb1[i] = *reinterpret_cast<const u32x4*>(Bg1 + o);
     });
   };
   auto lstore = [&]() {
-    static_for<0, VPT>([&](auto ic) {
+    static_for<0, VPA>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const int v = tid + i * kGThreads;
       const int o = (v / VPR) * LDK + (v % VPR) * EPV;
       *reinterpret_cast<u32x4*>(&As[0][o]) = ra0[i];
+      if constexpr (MODE == 2) *reinterpret_cast<u32x4*>(&As[NSPLIT - 1][o]) = ra1[i];
+    });
+    static_for<0, VPB>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int v = tid + i * kGThreads;
+      const int o = (v / VPR) * LDK + (v % VPR) * EPV;
       *reinterpret_cast<u32x4*>(&Bs[0][o]) = rb0[i];
-      if constexpr (MODE == 2) {
-        *reinterpret_cast<u32x4*>(&As[NSPLIT - 1][o]) = ra1[i];
-        *reinterpret_cast<u32x4*>(&Bs[NSPLIT - 1][o]) = rb1[i];
-      }
+      if constexpr (MODE == 2) *reinterpret_cast<u32x4*>(&Bs[NSPLIT - 1][o]) = rb1[i];
     });
   };
 
-  f32x16 acc[2][2];
+  f32x16 acc[TM][TN];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int c = 0; c < 2; c++)
+    for (int c = 0; c < TN; c++)
 #pragma unroll
       for (int v = 0; v < 16; v++) acc[a][c][v] = 0.f;
 
@@ -173,31 +184,33 @@ __global__ __launch_bounds__(kGThreads) void k_gemm(const typename GemmCfg<MODE>
 #pragma unroll
     for (int ks = 0; ks < BK; ks += KSTEP) {
       if constexpr (MODE == 0) {
-        float af[2], bf[2];
+        float af[TM], bf[TN];
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-          af[t] = As[0][(wm * 64 + t * 32 + fr) * LDK + ks + fk];
-          bf[t] = Bs[0][(wn * 64 + t * 32 + fr) * LDK + ks + fk];
-        }
+        for (int t = 0; t < TM; t++) af[t] = As[0][(wm * 32 * TM + t * 32 + fr) * LDK + ks + fk];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int t = 0; t < TN; t++) bf[t] = Bs[0][(wn * 32 * TN + t * 32 + fr) * LDK + ks + fk];
 #pragma unroll
-          for (int c = 0; c < 2; c++) acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[a], bf[c], acc[a][c], 0, 0, 0);
+        for (int a = 0; a < TM; a++)
+#pragma unroll
+          for (int c = 0; c < TN; c++) acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[a], bf[c], acc[a][c], 0, 0, 0);
       } else {
-        bf16x8 ah[2], bh[2], al[2], bl[2];
+        bf16x8 ah[TM], bh[TN], al[TM], bl[TN];
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-          ah[t] = *reinterpret_cast<const bf16x8*>(&As[0][(wm * 64 + t * 32 + fr) * LDK + ks + fk * 8]);
-          bh[t] = *reinterpret_cast<const bf16x8*>(&Bs[0][(wn * 64 + t * 32 + fr) * LDK + ks + fk * 8]);
-          if constexpr (MODE == 2) {
-            al[t] = *reinterpret_cast<const bf16x8*>(&As[NSPLIT - 1][(wm * 64 + t * 32 + fr) * LDK + ks + fk * 8]);
-            bl[t] = *reinterpret_cast<const bf16x8*>(&Bs[NSPLIT - 1][(wn * 64 + t * 32 + fr) * LDK + ks + fk * 8]);
-          }
+        for (int t = 0; t < TM; t++) {
+          ah[t] = *reinterpret_cast<const bf16x8*>(&As[0][(wm * 32 * TM + t * 32 + fr) * LDK + ks + fk * 8]);
+          if constexpr (MODE == 2)
+            al[t] = *reinterpret_cast<const bf16x8*>(&As[NSPLIT - 1][(wm * 32 * TM + t * 32 + fr) * LDK + ks + fk * 8]);
         }
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+        for (int t = 0; t < TN; t++) {
+          bh[t] = *reinterpret_cast<const bf16x8*>(&Bs[0][(wn * 32 * TN + t * 32 + fr) * LDK + ks + fk * 8]);
+          if constexpr (MODE == 2)
+            bl[t] = *reinterpret_cast<const bf16x8*>(&Bs[NSPLIT - 1][(wn * 32 * TN + t * 32 + fr) * LDK + ks + fk * 8]);
+        }
 #pragma unroll
-          for (int c = 0; c < 2; c++) {
+        for (int a = 0; a < TM; a++)
+#pragma unroll
+          for (int c = 0; c < TN; c++) {
             if constexpr (MODE == 2) {
               acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
               acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
@@ -214,14 +227,14 @@ __global__ __launch_bounds__(kGThreads) void k_gemm(const typename GemmCfg<MODE>
   }
   // accumulator layout of a 32 x 32 MFMA tile: lane l holds column l % 32, rows 8 * (v / 4) + 4 * (l / 32) + v % 4
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int c = 0; c < 2; c++) {
-      const int64_t col = n0 + wn * 64 + c * 32 + fr;
+    for (int c = 0; c < TN; c++) {
+      const int64_t col = n0 + wn * 32 * TN + c * 32 + fr;
       if (col >= N) continue;
 #pragma unroll
       for (int v = 0; v < 16; v++) {
-        const int64_t row = m0 + wm * 64 + a * 32 + 8 * (v / 4) + 4 * fk + (v % 4);
+        const int64_t row = m0 + wm * 32 * TM + a * 32 + 8 * (v / 4) + 4 * fk + (v % 4);
         if (row < M) {
           float* p = C + row * ldc + col;
           *p = beta == 0.f ? alpha * acc[a][c][v] : fmaf(beta, *p, alpha * acc[a][c][v]);
@@ -248,12 +261,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int64_t M, int64_t N, int64_t 
   if (row < M && col < N) C[row * ldc + col] += s;
 }
 
-template <int MODE>
-static void gemm_impl(bclx_handle* h, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const float* A,
-                      int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc) {
+// Wavefront tile (TM x TN MFMA tiles) per mode: BF16 uses 4 x 2 (block 256 x 128) when M is large enough to fill the
+// GPU with such blocks; FP32 and split-BF16 (twice the LDS) keep 2 x 2 (block 128 x 128).
+template <int MODE, int TM, int TN>
+static void gemm_run(bclx_handle* h, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const float* A,
+                     int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc) {
   using E = typename GemmCfg<MODE>::E;
-  constexpr int BK = GemmCfg<MODE>::BK;
-  const int64_t Mp = (m + kGBM - 1) / kGBM * kGBM, Np = (n + kGBN - 1) / kGBN * kGBN;
+  constexpr int BK = GemmCfg<MODE>::BK, BM = 64 * TM, BN = 64 * TN;
+  const int64_t Mp = (m + BM - 1) / BM * BM, Np = (n + BN - 1) / BN * BN;
   const int64_t Kp = std::max<int64_t>(BK, (k + BK - 1) / BK * BK);
   const size_t a_bytes = (size_t)Mp * Kp * sizeof(E), b_bytes = (size_t)Np * Kp * sizeof(E);
   const size_t lo_a = MODE == 2 ? (size_t)Mp * Kp * 2 : 0, lo_b = MODE == 2 ? (size_t)Np * Kp * 2 : 0;
@@ -273,10 +288,18 @@ static void gemm_impl(bclx_handle* h, bool ta, bool tb, int64_t m, int64_t n, in
   hipLaunchKernelGGL(k_pack<MODE>, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), blk, 0, h->stream, B, n, k,
                      tb ? ldb : (int64_t)1, tb ? (int64_t)1 : ldb, (void*)Bp, Bl, Np, Kp);
   BCLX_HIP(hipGetLastError());
-  const int tiles_m = (int)(Mp / kGBM), tiles_n = (int)(Np / kGBN);
-  hipLaunchKernelGGL(k_gemm<MODE>, dim3((unsigned)(tiles_m * tiles_n)), dim3(kGThreads), 0, h->stream, Ap, (const E*)Al,
-                     Bp, (const E*)Bl, m, n, Kp, alpha, beta, C, ldc, tiles_m, tiles_n);
+  const int tiles_m = (int)(Mp / BM), tiles_n = (int)(Np / BN);
+  hipLaunchKernelGGL((k_gemm<MODE, TM, TN>), dim3((unsigned)(tiles_m * tiles_n)), dim3(kGThreads), 0, h->stream, Ap,
+                     (const E*)Al, Bp, (const E*)Bl, m, n, Kp, alpha, beta, C, ldc, tiles_m, tiles_n);
   BCLX_HIP(hipGetLastError());
+}
+
+template <int MODE>
+static void gemm_impl(bclx_handle* h, bool ta, bool tb, int64_t m, int64_t n, int64_t k, float alpha, const float* A,
+                      int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc) {
+  const bool big = MODE == 1 && ((m + 255) / 256) * ((n + 127) / 128) >= 2 * (int64_t)h->num_cus;
+  if (big) gemm_run<MODE, 4, 2>(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  else gemm_run<MODE, 2, 2>(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 }  // namespace bclx
