@@ -101,7 +101,7 @@ typedef struct {
   float margin;             /* predicted capacity = ceil(pred * (1 + margin)) + slack */
   int32_t slack;
   const float* h_mlp_params; /* 5313 fp32 MLP parameters (NULL: mode 1 falls back to upper bounds) */
-  int32_t persistent_blocks; /* 0 = automatic */
+  int32_t persistent_blocks; /* reserved: pass 0 (grids are sized from the plan's own bin counts) */
   const int32_t* h_tuning;   /* optional BCLX_TUNING_LEN binning knobs (autotuned), NULL = defaults:
                               *   [0] work_per_lane (64)  lanes per row >= products / work_per_lane
                               *   [1] heavy_need (4096)   rows needing more slots use the heavy accumulator
