@@ -235,16 +235,17 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
       case 0: launch_esc<1>(R, s, mode, b, grid); break;
       case 1: launch_esc<2>(R, s, mode, b, grid); break;
       case 2: launch_esc<4>(R, s, mode, b, grid); break;
-      default: launch_esc<8>(R, s, mode, b, grid); break;
+      case 3: launch_esc<8>(R, s, mode, b, grid); break;
+      default: launch_esc<16>(R, s, mode, b, grid); break;
     }
     return;
   }
   if (b >= kEscBin0 + kEscWaveBins) {
     unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, count_hint) : (unsigned)(h->num_cus * 8);
     grid = std::min<unsigned>(grid, 1u << 20);
-    switch (b - kEscBin0 - kEscWaveBins) {
-      case 0: launch_esc_block<2>(R, s, mode, b, grid); break;
-      case 1: launch_esc_block<4>(R, s, mode, b, grid); break;
+    switch ((64 << (b - kEscBin0)) / kEscChunk) {  // wavefronts per row
+      case 2: launch_esc_block<2>(R, s, mode, b, grid); break;
+      case 4: launch_esc_block<4>(R, s, mode, b, grid); break;
       default: launch_esc_block<8>(R, s, mode, b, grid); break;
     }
     return;
