@@ -741,6 +741,39 @@ __device__ inline void esc_sort(unsigned (&key)[K], int lane) {
   }
 }
 
+// Half-cleaner stages at distances 32 * K .. 1 (ascending) over one wavefront's 64 * K keys, which merge a bitonic
+// sequence into sorted order: in-register stages for distances of 64 or more, lane exchanges below.
+template <int K>
+__device__ inline void esc_half_clean(unsigned (&key)[K], int lane) {
+#pragma unroll
+  for (int d = 32 * K; d > 0; d >>= 1) {
+    if (d >= 64) {
+      const int dk = d >> 6;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if ((k & dk) == 0) {
+          const unsigned a = key[k], b = key[k | dk];
+          key[k] = min(a, b);
+          key[k | dk] = max(a, b);
+        }
+      }
+    } else {
+      const bool keep_min = (lane & d) == 0;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        switch (d) {
+          case 1: key[k] = esc_xchg<1>(key[k], keep_min, lane); break;
+          case 2: key[k] = esc_xchg<2>(key[k], keep_min, lane); break;
+          case 4: key[k] = esc_xchg<4>(key[k], keep_min, lane); break;
+          case 8: key[k] = esc_xchg<8>(key[k], keep_min, lane); break;
+          case 16: key[k] = esc_xchg<16>(key[k], keep_min, lane); break;
+          default: key[k] = esc_xchg<32>(key[k], keep_min, lane); break;
+        }
+      }
+    }
+  }
+}
+
 template <int K, int MODE, typename T, typename RP>
 __global__ __launch_bounds__(64 * kEscWaves) void k_esc(RowsDev<T, RP> R, int bin) {
   constexpr int N = 64 * K;
@@ -929,41 +962,34 @@ __global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int b
       __syncthreads();
     }
     esc_sort<K>(key, lane);
-    // thread (w, lane) owns positions w * 512 + k * 64 + lane in every round
+    // merge the wavefronts' sorted runs with the rest of a bitonic network: for each block size S = 1024 .. N, a
+    // mirror stage (position E against E ^ (S - 1)) and half-cleaners at distances S/4 .. 1. Distances of 512 or
+    // more cross wavefronts and go through LDS; shorter ones stay in registers and lanes.
+    const int E0 = w * kEscChunk + lane;  // position of key[k] is E0 + k * 64
+    // one compare-exchange stage across wavefronts: partner position E ^ d through LDS; mirror stages (d = S - 1)
+    // keep the minimum in the lower half of each S-block, half-cleaners in the lower half of each 2d-block
+    auto lds_stage = [&](int d, int S) {
 #pragma unroll
-    for (int k = 0; k < K; k++) s_key[w * kEscChunk + k * 64 + lane] = key[k];
-    __syncthreads();
-#pragma unroll
-    for (int L = kEscChunk; L < N; L <<= 1) {
-      // a wavefront's 512 positions lie in one run (L is a multiple of 512), so its partner run is uniform
-      const int run = (w * kEscChunk) / L;
-      const int other = (run ^ 1) * L;
-      const bool left = (run & 1) == 0;
-      const int base_out = (run & ~1) * L + (w * kEscChunk - run * L) + lane;  // + k * 64 + rank
-      // rank of each key in the partner run (left run: partner keys < key; right run: partner keys <= key), all K
-      // binary searches in lockstep with power-of-two steps so their LDS latencies overlap
-      int rank[K];
-#pragma unroll
-      for (int k = 0; k < K; k++) rank[k] = 0;
-      for (int step = L >> 1; step > 0; step >>= 1) {
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-          const unsigned o = s_key[other + rank[k] + step - 1];
-          if (o < key[k] || (!left && o == key[k])) rank[k] += step;
-        }
-      }
+      for (int k = 0; k < K; k++) s_key[E0 + k * 64] = key[k];
+      __syncthreads();
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const unsigned o = s_key[other + rank[k]];
-        if (o < key[k] || (!left && o == key[k])) rank[k] += 1;
+        const int E = E0 + k * 64;
+        const unsigned o = s_key[E ^ d];
+        const bool lower = d == S - 1 ? (E & (S >> 1)) == 0 : (E & d) == 0;
+        key[k] = lower ? min(key[k], o) : max(key[k], o);
       }
       __syncthreads();
+    };
 #pragma unroll
-      for (int k = 0; k < K; k++) s_key[base_out + k * 64 + rank[k]] = key[k];
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < K; k++) key[k] = s_key[w * kEscChunk + k * 64 + lane];
+    for (int S = 2 * kEscChunk; S <= N; S <<= 1) {
+      lds_stage(S - 1, S);
+      for (int d = S >> 2; d >= kEscChunk; d >>= 1) lds_stage(d, S);
+      esc_half_clean<K>(key, lane);
     }
+#pragma unroll
+    for (int k = 0; k < K; k++) s_key[E0 + k * 64] = key[k];
+    __syncthreads();
     // compress: wavefront w owns sorted positions [w * 512, (w + 1) * 512), as k * 64 + lane
     int wcnt = 0;
 #pragma unroll
