@@ -79,7 +79,7 @@ void FlatRows<T, RP>::setup(bclx_handle* h_, const std::vector<TileDev<T, RP>>& 
   size_t o_nnz = A.reserve(sizeof(int32_t) * nflat);
   size_t o_cap = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
   size_t o_off = A.reserve(sizeof(int64_t) * (nflat + 1));
-  size_t o_bc = A.reserve(sizeof(int32_t) * kNumBins);
+  size_t o_bc = A.reserve(sizeof(int32_t) * kNumBins * kCountStride);
   size_t o_bi = A.reserve(sizeof(int32_t) * kNumBins * nflat);
   size_t o_oc = A.reserve(sizeof(int32_t) * 2);
   size_t o_oi = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
@@ -125,21 +125,21 @@ void FlatRows<T, RP>::upload_tiles() {
 
 template <typename T, typename RP>
 void FlatRows<T, RP>::reset_counters(hipStream_t s) {
-  fill_i32(s, R.bin_count, kNumBins, 0);
+  fill_i32(s, R.bin_count, kNumBins * kCountStride, 0);
   fill_i32(s, R.ovf_count, 2, 0);
   fill_u64(s, R.stats, 8, 0);
 }
 
 template <typename T, typename RP>
-void FlatRows<T, RP>::analyze(hipStream_t s, bool do_bin, long long pred_min) {
+void FlatRows<T, RP>::analyze(hipStream_t s, bool do_bin, long long pred_min, bool with_stats) {
   hipLaunchKernelGGL((k_analyze<T, RP>), dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, R, do_bin ? 1 : 0,
-                     pred_min);
+                     pred_min, with_stats ? 1 : 0);
   BCLX_HIP(hipGetLastError());
 }
 
 template <typename T, typename RP>
 void FlatRows<T, RP>::bin(hipStream_t s, int src) {
-  fill_i32(s, R.bin_count, kNumBins, 0);
+  fill_i32(s, R.bin_count, kNumBins * kCountStride, 0);
   hipLaunchKernelGGL((k_bin<T, RP>), dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, R, src);
   BCLX_HIP(hipGetLastError());
 }
@@ -325,6 +325,14 @@ void FlatRows<T, RP>::compact(hipStream_t s, const int64_t* dst_off, int* dst_co
 }
 
 template <typename T, typename RP>
+void FlatRows<T, RP>::read_counts(hipStream_t s, int32_t* dst) {
+  int32_t raw[kNumBins * kCountStride];
+  BCLX_HIP(hipMemcpyAsync(raw, R.bin_count, sizeof(raw), hipMemcpyDeviceToHost, s));
+  BCLX_HIP(hipStreamSynchronize(s));
+  for (int b = 0; b < kNumBins; b++) dst[b] = raw[b * kCountStride];
+}
+
+template <typename T, typename RP>
 int64_t FlatRows<T, RP>::read_stat(int q) {
   unsigned long long v = 0;
   BCLX_HIP(hipMemcpyAsync(&v, R.stats + q, sizeof(v), hipMemcpyDeviceToHost, h->stream));
@@ -397,8 +405,7 @@ int64_t TwoPhase<T, RP>::symbolic() {
   F.reset_counters(s);
   F.analyze(s, true);
   int32_t counts[kNumBins];
-  BCLX_HIP(hipMemcpyAsync(counts, F.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
-  BCLX_HIP(hipStreamSynchronize(s));
+  F.read_counts(s, counts);
   if (counts[kHeavyBin] > 0) F.prepare_heavy();
   for (int b = 1; b < kNumBins; b++) F.run_bin(s, 0, b, counts[b]);
   F.scan(s, F.R.nnz, F.R.off, 0);
@@ -424,8 +431,7 @@ void TwoPhase<T, RP>::numeric(int* c_col, T* c_val, int64_t c_cap, void* row_ptr
   F.R.out_limit = c_cap;
   F.bin(s, 0);
   int32_t counts[kNumBins];
-  BCLX_HIP(hipMemcpyAsync(counts, F.R.bin_count, sizeof(counts), hipMemcpyDeviceToHost, s));
-  BCLX_HIP(hipStreamSynchronize(s));
+  F.read_counts(s, counts);
   for (int b = 1; b < kNumBins; b++) F.run_bin(s, 1, b, counts[b]);
   const unsigned g = div_up(m + 1, 256);
   if (ptr_width == 8)

@@ -39,7 +39,8 @@ struct FlatRows {
   void upload_tiles();
   void reset_counters(hipStream_t s);
   // pred_min >= 0 (learned path): also write cap = min(prod, n) and list the rows with more than pred_min products
-  void analyze(hipStream_t s, bool do_bin, long long pred_min = -1);
+  // with_stats: also count products (stats[0]) and rows bound for the heavy bin (stats[5])
+  void analyze(hipStream_t s, bool do_bin, long long pred_min = -1, bool with_stats = true);
   void bin(hipStream_t s, int src);
   // exclusive scan of cnt over all flat rows into off (nflat + 1 entries), plus off_add; total -> *scan_total
   void scan(hipStream_t s, const int32_t* cnt, int64_t* off, int64_t off_add);
@@ -50,6 +51,7 @@ struct FlatRows {
   // heavy rows: allocate the zeroed accumulators of the heavy kernel (before any graph capture)
   void prepare_heavy();
   int64_t read_stat(int q);
+  void read_counts(hipStream_t s, int32_t* dst);  // kNumBins bin counts (synchronises s)
 };
 
 // The two-phase product C = A * B of one tile pair (symbolic, then numeric into caller-allocated C).

@@ -17,6 +17,7 @@ constexpr int kEmptyKey = 0x7fffffff;  // larger than any column index
 // 64 * 2^(b - 11) products (bins 11..14: one wavefront per row; 15..17: 2, 4 or 8 wavefronts per row)
 constexpr int kNumBins = 18;
 constexpr int kHeavyBin = 10;
+constexpr int kCountStride = 32;  // bin counters sit 128 bytes apart
 constexpr int kEscBin0 = 11;
 constexpr int kEscBins = 7;
 constexpr int kEscWaveBins = 4;                         // bins 11..14 sort within one wavefront
@@ -95,7 +96,9 @@ struct RowsDev {
   int32_t* nnz;              // nnz(C_i): symbolic count, or measured count
   int32_t* cap;              // predicted capacity (learned path)
   int64_t* off;              // output offset per flat row (nflat + 1 entries)
-  int32_t* bin_count;        // kNumBins counters
+  int32_t* bin_count;        // kNumBins counters, kCountStride ints apart (one cache line each: no false sharing of
+                             // the atomics that fill them)
+  __host__ __device__ int32_t* count_of(int b) const { return bin_count + b * kCountStride; }
   int32_t* bin_items;        // kNumBins lists of flat rows, bin_stride apart
   int64_t bin_stride;
   int32_t* ovf_count;        // overflow list (learned path)
@@ -169,7 +172,8 @@ __device__ inline V block_sum(V v, V* smem) {
 // ------------------------------------------------------------------------------------------------------------
 
 template <typename T, typename RP>
-__global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_bin, long long pred_min) {
+__global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_bin, long long pred_min,
+                                                       int with_stats) {
   __shared__ int s_long[kRowBlock];
   __shared__ int s_nlong;
   __shared__ int s_bcount[kNumBins], s_bbase[kNumBins];
@@ -220,7 +224,7 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_
   R.prod[fr] = valid ? prod : 0;
   const long long ub0 = prod < (long long)tile.n ? prod : (long long)tile.n;
   const int b0 = valid ? bin_for(ub0, prod, R.tn, tile.n <= (1 << kEscColBits)) : 0;
-  if (b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
+  if (with_stats && b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
   bool pr = false;
   if (pred_min >= 0) {
     // learned path: every row starts at the exact bound min(prod, n); rows with more than pred_min products that
@@ -240,13 +244,15 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_
     int pos = 0;
     if (b > 0) pos = atomicAdd(&s_bcount[b], 1);
     __syncthreads();
-    if (tid > 0 && tid < kNumBins && s_bcount[tid] > 0) s_bbase[tid] = atomicAdd(&R.bin_count[tid], s_bcount[tid]);
+    if (tid > 0 && tid < kNumBins && s_bcount[tid] > 0) s_bbase[tid] = atomicAdd(R.count_of(tid), s_bcount[tid]);
     __syncthreads();
     if (b > 0) R.bin_items[b * R.bin_stride + s_bbase[b] + pos] = (int)fr;
     if (b == 0 && !pr) R.nnz[fr] = 0;  // empty and padding rows
   }
-  long long tot = block_sum<kRowBlock, long long>(prod, s_red);
-  if (tid == 0 && tot) atomicAdd(&R.stats[0], (unsigned long long)tot);
+  if (with_stats) {  // one global atomic per block: only when asked (a plan counts products once)
+    long long tot = block_sum<kRowBlock, long long>(prod, s_red);
+    if (tid == 0 && tot) atomicAdd(&R.stats[0], (unsigned long long)tot);
+  }
 }
 
 // Bin rows by `need` (src 0: nnz, 1: cap) and products. Padded / empty rows go nowhere.
@@ -269,7 +275,7 @@ __global__ __launch_bounds__(kRowBlock) void k_bin(RowsDev<T, RP> R, int src) {
   int pos = 0;
   if (b > 0) pos = atomicAdd(&s_bcount[b], 1);
   __syncthreads();
-  if (tid > 0 && tid < kNumBins && s_bcount[tid] > 0) s_bbase[tid] = atomicAdd(&R.bin_count[tid], s_bcount[tid]);
+  if (tid > 0 && tid < kNumBins && s_bcount[tid] > 0) s_bbase[tid] = atomicAdd(R.count_of(tid), s_bcount[tid]);
   __syncthreads();
   if (b > 0) R.bin_items[b * R.bin_stride + s_bbase[b] + pos] = (int)fr;
 }
@@ -468,7 +474,7 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
   const int g = threadIdx.x / G, lane = threadIdx.x % G;
   int* keys = reinterpret_cast<int*>(s_buf) + g * TS;
   double* vals = reinterpret_cast<double*>(s_buf + NG * TS * 4) + (NUM ? g * TS : 0);
-  const int count = R.bin_count[bin];
+  const int count = *R.count_of(bin);
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
   volatile int* vflag = s_flag;
 
@@ -784,7 +790,7 @@ __global__ __launch_bounds__(64 * kEscWaves) void k_esc(RowsDev<T, RP> R, int bi
   __shared__ RP s_st[kEscWaves][64];                // B-row start of each entry
   __shared__ T s_av[kEscWaves][64];                 // A value of each entry
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int count = R.bin_count[bin];
+  const int count = *R.count_of(bin);
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
   for (int idx = blockIdx.x * kEscWaves + w; idx < count; idx += gridDim.x * kEscWaves) {
     const int fr = items[idx];
@@ -912,7 +918,7 @@ __global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int b
   __shared__ int s_cnt[W];
   __shared__ long long s_o;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int count = R.bin_count[bin];
+  const int count = *R.count_of(bin);
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
   for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
     const int fr = items[idx];
@@ -1129,7 +1135,7 @@ __device__ inline RP hr_lower_bound(const int* b_col, RP st, RP len, int c) {
 template <typename T, typename RP>
 __global__ __launch_bounds__(1024) void k_order_heavy(RowsDev<T, RP> R) {
   __shared__ int s_hist[32], s_next[32];
-  const int cnt = R.bin_count[kHeavyBin];
+  const int cnt = *R.count_of(kHeavyBin);
   int* items = R.bin_items + (int64_t)kHeavyBin * R.bin_stride;
   if (threadIdx.x < 32) s_hist[threadIdx.x] = 0;
   __syncthreads();
@@ -1173,7 +1179,7 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
   __shared__ long long s_lscan[BLOCK / 64 + 1];
   constexpr int kGroupsPerThread = (kHRWords / kHRGroup + BLOCK - 1) / BLOCK;
   const int tid = threadIdx.x;
-  const int count = R.bin_count[bin];
+  const int count = *R.count_of(bin);
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
   int* s_colbuf = reinterpret_cast<int*>(s_acc);
   __shared__ int s_idx;
@@ -1373,7 +1379,7 @@ __global__ __launch_bounds__(kHeavyThreads) void k_heavy(RowsDev<T, RP> R, int b
   const int tid = threadIdx.x;
   double* dense = dense_slots + (int64_t)blockIdx.x * slot_doubles;
   unsigned* bits = LDS_BITMAP ? s_dyn : gbits + (int64_t)blockIdx.x * words;
-  const int count = R.bin_count[bin];
+  const int count = *R.count_of(bin);
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
   for (int idx = blockIdx.x; idx < count; idx += gridDim.x) {
     const int fr = items[idx];

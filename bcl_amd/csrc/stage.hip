@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_bin_predicted(RowsDev<T, RP> R) {
     }
     __syncthreads();
     if (threadIdx.x < kNumBins && s_cnt[threadIdx.x] > 0)
-      s_base[threadIdx.x] = atomicAdd(&R.bin_count[threadIdx.x], s_cnt[threadIdx.x]);
+      s_base[threadIdx.x] = atomicAdd(R.count_of(threadIdx.x), s_cnt[threadIdx.x]);
     __syncthreads();
     if (idx < cnt) R.bin_items[(int64_t)b * R.bin_stride + s_base[b] + pos] = fr;
     __syncthreads();
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_ovf_bin(RowsDev<T, RP> R) {
     const long long prod = R.prod[fr];
     const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
     const int b = bin_for(ub, prod, R.tn, tile.n <= (1 << kEscColBits));
-    const int pos = atomicAdd(&R.bin_count[b], 1);
+    const int pos = atomicAdd(R.count_of(b), 1);
     R.bin_items[(int64_t)b * R.bin_stride + pos] = fr;
   }
 }
@@ -225,11 +225,11 @@ struct StagePlan : StageBase {
 
   // Analysis with capacities (exact bound everywhere) and binning of every row that is not predicted, then the
   // network on the listed rows (when weights are given) and their binning by the predicted capacity.
-  void predict(hipStream_t s) {
+  void predict(hipStream_t s, bool with_stats) {
     const bool use_mlp = opts.h_mlp_params != nullptr;
-    fill_i32(s, F.R.bin_count, kNumBins, 0);
+    fill_i32(s, F.R.bin_count, kNumBins * kCountStride, 0);
     fill_i32(s, F.R.pred_count, 1, 0);
-    F.analyze(s, true, use_mlp ? (long long)kPredictMinProducts : LLONG_MAX);
+    F.analyze(s, true, use_mlp ? (long long)kPredictMinProducts : LLONG_MAX, with_stats);
     if (use_mlp) {
       const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(div_up(F.nflat, kPredictBlock),
                                                                             (int64_t)4 * h->num_cus));
@@ -245,10 +245,10 @@ struct StagePlan : StageBase {
     if (mode == 1) {
       fill_i32(s, F.R.nnz, F.nflat, 0);
       fill_u64(s, F.R.cursor, 1, (unsigned long long)main_cap);
-      predict(s);
+      predict(s, false);  // products were counted by the sizing run
       F.scan(s, F.R.cap, F.R.off, 0);
       run_bins(s, 2, counts1, persistent);
-      fill_i32(s, F.R.bin_count, kNumBins, 0);
+      fill_i32(s, F.R.bin_count, kNumBins * kCountStride, 0);
       hipLaunchKernelGGL((k_ovf_bin<T, RP>), dim3(std::max<unsigned>(1, std::min<unsigned>(div_up(counts_ovf, 256), 1024))),
                          dim3(256), 0, s, F.R);
       BCLX_HIP(hipGetLastError());
@@ -257,7 +257,7 @@ struct StagePlan : StageBase {
       F.compact(s, final_off.as<int64_t>(), final_col.as<int>(), final_val.as<T>(), arena_col.as<int>(),
                 arena_val.as<T>(), final_cap, chunk_row.as<int32_t>());
     } else {
-      F.analyze(s, true);
+      F.analyze(s, true, -1, false);
       run_bins(s, 0, counts1, persistent);
       F.scan(s, F.R.nnz, F.R.off, 0);
       F.bin(s, 0);
@@ -271,17 +271,18 @@ struct StagePlan : StageBase {
 
   unsigned counts_ovf = 0;
   bool has_heavy = false;
+  int64_t products = 0;
 
   // after the sizing run's analysis: window indexes and scratch for the rows that can take the heavy path
   void setup_heavy(hipStream_t s) {
     (void)s;
+    products = F.read_stat(0);  // the sizing run's analysis counts products; graph replays do not
     has_heavy = F.read_stat(5) > 0;
     if (has_heavy) F.prepare_heavy();
   }
 
   void read_counts(int32_t* dst) {
-    BCLX_HIP(hipMemcpyAsync(dst, F.R.bin_count, sizeof(int32_t) * kNumBins, hipMemcpyDeviceToHost, h->stream));
-    BCLX_HIP(hipStreamSynchronize(h->stream));
+    F.read_counts(h->stream, dst);
   }
 
   int64_t read_stat(int q) {
@@ -297,7 +298,7 @@ struct StagePlan : StageBase {
     F.reset_counters(s);
     if (mode == 1) {
       fill_i32(s, F.R.nnz, F.nflat, 0);
-      predict(s);
+      predict(s, true);
       setup_heavy(s);
       F.scan(s, F.R.cap, F.R.off, 0);
       int64_t tot_cap = 0;
@@ -319,7 +320,7 @@ struct StagePlan : StageBase {
         F.R.out_limit = arena_cap;
         if (attempt > 0) {  // the previous attempt's overflow redo re-binned: analyse and bin again (same capacities)
           F.reset_counters(s);
-          predict(s);
+          predict(s, false);
         }
         fill_u64(s, F.R.cursor, 1, (unsigned long long)main_cap);
         fill_u64(s, F.R.stats + 2, 2, 0);
@@ -331,7 +332,7 @@ struct StagePlan : StageBase {
         BCLX_HIP(hipMemcpyAsync(&novf, F.R.ovf_count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         BCLX_HIP(hipStreamSynchronize(s));
         counts_ovf = (unsigned)novf;
-        fill_i32(s, F.R.bin_count, kNumBins, 0);
+        fill_i32(s, F.R.bin_count, kNumBins * kCountStride, 0);
         hipLaunchKernelGGL((k_ovf_bin<T, RP>), dim3(std::max<unsigned>(1, std::min<unsigned>(div_up(novf, 256), 1024))),
                            dim3(256), 0, s, F.R);
         BCLX_HIP(hipGetLastError());
@@ -462,7 +463,7 @@ struct StagePlan : StageBase {
       const int64_t* off = mode == 1 ? final_off.as<int64_t>() : F.R.off;
       int64_t tot = 0;
       BCLX_HIP(hipMemcpy(&tot, off + F.nflat, sizeof(int64_t), hipMemcpyDeviceToHost));
-      st[0] = (int64_t)v[0];
+      st[0] = products;
       st[1] = tot;
       st[2] = (int64_t)v[2];
       st[3] = (int64_t)v[4];
