@@ -1064,8 +1064,9 @@ __global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int b
 // ------------------------------------------------------------------------------------------------------------
 // Heavy rows (more products than the sort-based bins take and more output entries than the largest hash table),
 // rank-slice accumulator: one workgroup per row, everything in LDS.
-//   pass 1: LDS bitmap of the row's columns (one atomicOr per product) and prefix counts per 16-word group, so every
-//           column's output rank is known and nnz(C_i) is exact; the columns are written straight from the bitmap;
+//   pass 1: LDS bitmap of the row's columns (one atomicOr per product) and prefix counts per group of kHRGroup
+//           words, so every column's output rank is known and nnz(C_i) is exact; the columns are written straight
+//           from the bitmap;
 //   pass 2 (numeric modes), per slice of hr_slice() ranks: the products whose columns fall in the slice's column range
 //           are added in float64 into an LDS array indexed by rank - slice start (ds_add_f64), then written out with
 //           unit stride. Most rows fit one slice; larger rows enumerate their products once per slice.
@@ -1074,29 +1075,28 @@ __global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int b
 // ------------------------------------------------------------------------------------------------------------
 
 constexpr int kHRWords = 16384;  // bitmap words: tiles up to 2^19 columns
-constexpr int kHRGroup = 16;     // words per prefix group
-// float64 accumulators per slice: what LDS holds beside the bitmap and the entry staging (72 KB or 64 KB)
+// Words per prefix group: a column's rank is its group's prefix plus the bits before it in the group, read as one
+// 16-byte LDS load. Narrow groups cost prefix space (16 KB for 2^19 columns) but keep the per-product LDS reads at
+// 20 bytes; 16-word groups (four loads) were measured 0.75 ms slower on the s22 stage despite larger slices.
+constexpr int kHRGroup = 4;
+// float64 accumulators per slice: what LDS holds beside the bitmap, the prefixes and the entry staging (60 KB or 52 KB)
 template <typename T, typename RP>
-constexpr int hr_slice() { return sizeof(T) == 4 && sizeof(RP) == 4 ? 9216 : 8192; }
+constexpr int hr_slice() { return sizeof(T) == 4 && sizeof(RP) == 4 ? 7680 : 6656; }
 // symbolic: 512 threads, two workgroups per CU; numeric: 1024 threads, one workgroup per CU (bitmap + slice)
 template <int MODE>
 constexpr int hr_block() { return MODE == 0 ? 512 : 1024; }
 
 typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+static_assert(kHRGroup == 4, "hr_rank reads one 16-byte group");
 
-// Output rank of column j: set bits before j. The group prefix counts the bits before j's 16-word group; the words of
-// the group before j's word are read as four 16-byte LDS loads.
+// Output rank of column j: set bits before j. The group prefix counts the bits before j's group; the group (j's word
+// and the words before it) is one 16-byte LDS load.
 __device__ inline int hr_rank(const unsigned* bits, const int* gpre, int j) {
   const int w = j >> 5, g = w / kHRGroup, wi = w % kHRGroup;
-  const u32x4_t* q = reinterpret_cast<const u32x4_t*>(bits + g * kHRGroup);
-  int r = gpre[g];
-#pragma unroll
-  for (int v = 0; v < 4; v++) {
-    const u32x4_t x = q[v];
-    r += (4 * v + 0 < wi ? __popc(x.x) : 0) + (4 * v + 1 < wi ? __popc(x.y) : 0) + (4 * v + 2 < wi ? __popc(x.z) : 0) +
-         (4 * v + 3 < wi ? __popc(x.w) : 0);
-  }
-  return r + __popc(bits[w] & ((1u << (j & 31)) - 1u));
+  const u32x4_t x = *reinterpret_cast<const u32x4_t*>(bits + g * kHRGroup);
+  const unsigned below = (1u << (j & 31)) - 1u;
+  return gpre[g] + (wi > 0 ? __popc(x.x) : 0) + (wi > 1 ? __popc(x.y) : 0) + (wi > 2 ? __popc(x.z) : 0) +
+         __popc((wi == 0 ? x.x : wi == 1 ? x.y : wi == 2 ? x.z : x.w) & below);
 }
 
 // Column of output rank r (0 <= r < number of set bits): the r-th set bit. One thread.
@@ -1119,10 +1119,10 @@ __device__ inline int hr_select(const unsigned* bits, const int* gpre, int ng, i
   return w * 32 + (__ffs(word) - 1);
 }
 
-// First position in B row [st, st + len) whose column is >= c (binary search in global memory).
+// First position in [lo0, len) of the B row at st whose column is >= c (binary search in global memory).
 template <typename RP>
-__device__ inline RP hr_lower_bound(const int* b_col, RP st, RP len, int c) {
-  RP lo = 0, hi = len;
+__device__ inline RP hr_lower_bound(const int* b_col, RP st, RP lo0, RP len, int c) {
+  RP lo = lo0, hi = len;
   while (lo < hi) {
     const RP mid = (lo + hi) >> 1;
     if (b_col[st + mid] < c) lo = mid + 1; else hi = mid;
@@ -1168,7 +1168,7 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
   constexpr bool NUM = MODE != 0;
   constexpr int kHRSlice = hr_slice<T, RP>();
   __shared__ __align__(16) unsigned s_bits[kHRWords];
-  __shared__ int s_gpre[kHRWords / kHRGroup];
+  __shared__ int s_gpre[NUM ? kHRWords / kHRGroup : 1];  // the symbolic pass needs only the total
   __shared__ __align__(16) double s_acc[NUM ? kHRSlice : 1];  // a slice's columns (as int), then its values
   __shared__ long long s_scan[BLOCK / 64 + 1];
   __shared__ long long s_base;
@@ -1183,168 +1183,226 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
   int* s_colbuf = reinterpret_cast<int*>(s_acc);
   __shared__ int s_idx;
+  // Rows are taken from a queue (largest first), so one block's share is not fixed in advance. The next row is
+  // fetched while the current one runs: its queue slot when the current row starts, then one dependent load per
+  // barrier of the current row (bin item, tile, A row bounds, each entry's A column and value, its B-row range). A
+  // row then starts with its staging in registers instead of behind a chain of global round trips.
+  int nq = 0;        // thread 0: the next row's queue slot
+  int pst = 0;       // fetch steps done for the next row (uniform)
+  int n_idx = 0, n_fr = 0, n_tix = 0, n_kk = 0;
+  RP n_a0 = 0, n_a1 = 0, n_st = 0, n_len = 0;
+  T n_av = T(0);
+  auto advance = [&](int upto) {  // all threads, uniform upto; a barrier separates steps 1 and 2
+    if (pst < 1 && upto >= 1) {
+      if (tid == 0) s_idx = nq;
+      pst = 1;
+      if (upto >= 2) __syncthreads();
+    }
+    if (pst < 2 && upto >= 2) {
+      n_idx = __builtin_amdgcn_readfirstlane(s_idx);
+      if (n_idx < count) n_fr = items[n_idx];
+      pst = 2;
+    }
+    if (pst < 3 && upto >= 3) {
+      if (n_idx < count) n_tix = R.blk_tile[n_fr >> 8];
+      pst = 3;
+    }
+    if (pst < 4 && upto >= 4) {
+      if (n_idx < count) {
+        const TileDev<T, RP>& t = R.tiles[n_tix];
+        const int64_t ni = n_fr - t.flat0;
+        n_a0 = t.a_rp[ni];
+        n_a1 = t.a_rp[ni + 1];
+      }
+      pst = 4;
+    }
+    if (pst < 5 && upto >= 5) {
+      if (n_idx < count && n_a1 - n_a0 <= (RP)BLOCK && tid < (int)(n_a1 - n_a0)) {
+        const TileDev<T, RP>& t = R.tiles[n_tix];
+        n_kk = t.a_col[n_a0 + tid];
+        if (NUM) n_av = t.a_val[n_a0 + tid];
+      }
+      pst = 5;
+    }
+    if (pst < 6 && upto >= 6) {
+      if (n_idx < count && n_a1 - n_a0 <= (RP)BLOCK && tid < (int)(n_a1 - n_a0)) {
+        const TileDev<T, RP>& t = R.tiles[n_tix];
+        n_st = t.b_rp[n_kk];
+        n_len = t.b_rp[n_kk + 1] - n_st;
+      }
+      pst = 6;
+    }
+  };
+  if (tid == 0) nq = (int)atomicAdd(R.heavy_queue, 1ull);
+  advance(6);  // the first row: the whole chain, unhidden
   for (;;) {
-    // rows are taken from a queue (largest first), so one block's share is not fixed in advance
-    if (tid == 0) s_idx = (int)atomicAdd(R.heavy_queue, 1ull);
-    __syncthreads();
-    const int idx = s_idx;
-    __syncthreads();
-    if (idx >= count) break;
-    const int fr = items[idx];
-    const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
+    if (n_idx >= count) break;
+    const int fr = n_fr;
+    const TileDev<T, RP> tile = R.tiles[n_tix];
     const int64_t i = fr - tile.flat0;
     const int W = (tile.n + 31) / 32;
     const int Wr = (W + kHRGroup - 1) / kHRGroup * kHRGroup;  // whole groups (the tail words stay zero)
     for (int w = tid; w < Wr; w += BLOCK) s_bits[w] = 0u;
-    const RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
+    const RP a0 = n_a0, a1 = n_a1;
     const int na = (int)(a1 - a0);
     const bool one_chunk = na <= BLOCK;  // one entry per thread, staged once and kept in registers
-    RP my_st = 0, my_len = 0;
-    T my_av = T(0);
-    if (one_chunk && tid < na) {
-      const int kk = tile.a_col[a0 + tid];
-      my_st = tile.b_rp[kk];
-      my_len = tile.b_rp[kk + 1] - my_st;
-      if (NUM) my_av = tile.a_val[a0 + tid];
-    }
-    // stage (start, length) of every entry from registers and run the product loop
-    auto run_staged = [&](RP st, RP len, auto&& f, auto num_tag) {
-      constexpr bool kNum = decltype(num_tag)::value;
-      if (tid < na) {
-        s_est[tid] = st;
-        if (kNum) s_eval[tid] = my_av;
-      }
-      long long tot;
-      const long long ex = block_excl_scan<BLOCK, long long>((long long)(tid < na ? len : 0), &tot, s_lscan);
-      if (tid < na) s_epre[tid] = ex;
-      if (tid == 0) s_epre[na] = tot;
-      __syncthreads();
-      enumerate_staged<BLOCK, kNum, 4>(tile, na, tot, tid, s_est, s_epre, s_eval, nullptr, f);
-      __syncthreads();
-    };
-    __syncthreads();
-    auto set_bit = [&](int j, double) { atomicOr(&s_bits[j >> 5], 1u << (j & 31)); };
-    if (one_chunk) run_staged(my_st, my_len, set_bit, std::false_type{});
-    else for_each_product<BLOCK, BLOCK, false, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
-                                                      set_bit);
-    __syncthreads();
-    // group prefixes, in order: thread t counts groups [t * kGroupsPerThread, (t + 1) * kGroupsPerThread)
-    const int ng = Wr / kHRGroup;
-    int gc[kGroupsPerThread];
-    long long c = 0;
-#pragma unroll
-    for (int q = 0; q < kGroupsPerThread; q++) {
-      const int g = tid * kGroupsPerThread + q;
-      int n1 = 0;
-      if (g < ng) {
-        const u32x4_t* p4 = reinterpret_cast<const u32x4_t*>(s_bits + g * kHRGroup);
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const u32x4_t x = p4[v];
-          n1 += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+    const RP my_st = n_st, my_len = n_len;
+    const T my_av = n_av;
+    if (tid == 0) nq = (int)atomicAdd(R.heavy_queue, 1ull);
+    pst = 0;
+    n_st = 0;
+    n_len = 0;
+    n_av = T(0);
+    do {
+      // stage (start, length) of every entry from registers and run the product loop
+      auto run_staged = [&](RP st, RP len, auto&& f, auto num_tag) {
+        constexpr bool kNum = decltype(num_tag)::value;
+        if (tid < na) {
+          s_est[tid] = st;
+          if (kNum) s_eval[tid] = my_av;
         }
-      }
-      gc[q] = n1;
-      c += n1;
-    }
-    long long total;
-    long long run = block_excl_scan<BLOCK, long long>(c, &total, s_scan);
+        long long tot;
+        const long long ex = block_excl_scan<BLOCK, long long>((long long)(tid < na ? len : 0), &tot, s_lscan);
+        if (tid < na) s_epre[tid] = ex;
+        if (tid == 0) s_epre[na] = tot;
+        __syncthreads();
+        enumerate_staged<BLOCK, kNum, 4>(tile, na, tot, tid, s_est, s_epre, s_eval, nullptr, f);
+        __syncthreads();
+      };
+      __syncthreads();
+      auto set_bit = [&](int j, double) { atomicOr(&s_bits[j >> 5], 1u << (j & 31)); };
+      if (one_chunk) run_staged(my_st, my_len, set_bit, std::false_type{});
+      else for_each_product<BLOCK, BLOCK, false, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr,
+                                                        set_bit);
+      advance(1);
+      __syncthreads();
+      advance(2);
+      // group prefixes, in order: thread t counts groups [t * kGroupsPerThread, (t + 1) * kGroupsPerThread)
+      const int ng = Wr / kHRGroup;
+      int gc[kGroupsPerThread];
+      long long c = 0;
 #pragma unroll
-    for (int q = 0; q < kGroupsPerThread; q++) {
-      const int g = tid * kGroupsPerThread + q;
-      if (g < ng) s_gpre[g] = (int)run;
-      run += gc[q];
-    }
-    if (MODE == 0) {
+      for (int q = 0; q < kGroupsPerThread; q++) {
+        const int g = tid * kGroupsPerThread + q;
+        int n1 = 0;
+        if (g < ng) {
+          const u32x4_t* p4 = reinterpret_cast<const u32x4_t*>(s_bits + g * kHRGroup);
+#pragma unroll
+          for (int v = 0; v < kHRGroup / 4; v++) {
+            const u32x4_t x = p4[v];
+            n1 += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+          }
+        }
+        gc[q] = n1;
+        c += n1;
+      }
+      long long total;
+      long long run = block_excl_scan<BLOCK, long long>(c, &total, s_scan);
+      advance(3);
+#pragma unroll
+      for (int q = 0; q < kGroupsPerThread; q++) {
+        const int g = tid * kGroupsPerThread + q;
+        if (NUM && g < ng) s_gpre[g] = (int)run;
+        run += gc[q];
+      }
+      if (MODE == 0) {
+        if (tid == 0) {
+          R.nnz[fr] = (int)total;
+          atomicAdd(&R.stats[1], 1ull);
+        }
+        __syncthreads();
+        break;
+      }
       if (tid == 0) {
+        long long b;
+        if (MODE == 1 || (MODE == 2 && total <= (long long)R.cap[fr])) {
+          b = R.off[fr];
+        } else {
+          b = (long long)atomicAdd(R.cursor, (unsigned long long)total);
+          R.off[fr] = b;
+        }
+        if (b + total > R.out_limit) {
+          R.stats[3] = 1ull;
+          b = -1;
+          if (MODE != 1) R.off[fr] = -1;
+        }
+        s_base = b;
         R.nnz[fr] = (int)total;
         atomicAdd(&R.stats[1], 1ull);
       }
       __syncthreads();
-      continue;
-    }
-    if (tid == 0) {
-      long long b;
-      if (MODE == 1 || (MODE == 2 && total <= (long long)R.cap[fr])) {
-        b = R.off[fr];
-      } else {
-        b = (long long)atomicAdd(R.cursor, (unsigned long long)total);
-        R.off[fr] = b;
+      const long long base = s_base;
+      advance(4);
+      if (base < 0) {  // refused: nothing is written
+        __syncthreads();
+        break;
       }
-      if (b + total > R.out_limit) {
-        R.stats[3] = 1ull;
-        b = -1;
-        if (MODE != 1) R.off[fr] = -1;
-      }
-      s_base = b;
-      R.nnz[fr] = (int)total;
-      atomicAdd(&R.stats[1], 1ull);
-    }
-    __syncthreads();
-    const long long base = s_base;
-    if (base < 0) {  // refused: nothing is written
-      __syncthreads();
-      continue;
-    }
-    // one rank slice at a time: columns (staged in LDS by rank, then stored with unit stride), then values
-    for (long long s0 = 0; s0 < total; s0 += kHRSlice) {
-      const int len = (int)(total - s0 < kHRSlice ? total - s0 : kHRSlice);
-      const bool whole = s0 == 0 && len == total;
-      if (tid == 0) {  // column range of the slice
-        s_crange[0] = s0 == 0 ? 0 : hr_select(s_bits, s_gpre, ng, (int)s0);
-        s_crange[1] = s0 + len >= total ? INT_MAX : hr_select(s_bits, s_gpre, ng, (int)(s0 + len));
-      }
-      __syncthreads();
-      const int c_lo = s_crange[0], c_hi = s_crange[1];
-      const int w_lo = c_lo >> 5, w_hi = c_hi == INT_MAX ? W - 1 : (c_hi - 1) >> 5;
-      // a wavefront takes 64 consecutive words (four 16-word groups) at a time: each word's rank base is its group
-      // prefix plus the bits of the group's earlier words (a 16-lane scan of popcounts)
-      for (int wb = (w_lo & ~63) + (tid & ~63); wb <= w_hi; wb += BLOCK) {
-        const int lane = tid & 63, w = wb + lane;
-        const unsigned full = w < W ? s_bits[w] : 0u;
-        const int pc = __popc(full);
-        int inc = pc;
-#pragma unroll
-        for (int d = 1; d < kHRGroup; d <<= 1) {
-          const int t = __shfl_up(inc, d, kHRGroup);
-          if ((lane & (kHRGroup - 1)) >= d) inc += t;
+      // one rank slice at a time: columns (staged in LDS by rank, then stored with unit stride), then values. A
+      // slice's first column and each entry's first B position in it are where the previous slice ended.
+      RP my_lo = 0;
+      for (long long s0 = 0; s0 < total; s0 += kHRSlice) {
+        const int len = (int)(total - s0 < kHRSlice ? total - s0 : kHRSlice);
+        const bool whole = s0 == 0 && len == total;
+        if (tid == 0) {  // column range of the slice
+          s_crange[0] = s0 == 0 ? 0 : s_crange[1];
+          s_crange[1] = s0 + len >= total ? INT_MAX : hr_select(s_bits, s_gpre, ng, (int)(s0 + len));
         }
-        unsigned m = (w >= w_lo && w <= w_hi) ? full : 0u;
-        if (w == w_lo) m &= ~((1u << (c_lo & 31)) - 1u);
-        if (c_hi != INT_MAX && w == (c_hi >> 5)) m &= (1u << (c_hi & 31)) - 1u;
-        if (m) {
-          const int r0 = s_gpre[w / kHRGroup] + inc - pc - (int)s0;
-          while (m) {
-            const int b = __ffs(m) - 1;
-            s_colbuf[r0 + __popc(full & ((1u << b) - 1u))] = w * 32 + b;
-            m &= m - 1;
+        __syncthreads();
+        const int c_lo = s_crange[0], c_hi = s_crange[1];
+        const int w_lo = c_lo >> 5, w_hi = c_hi == INT_MAX ? W - 1 : (c_hi - 1) >> 5;
+        // a wavefront takes 64 consecutive words (whole groups) at a time: each word's rank base is its group prefix
+        // plus the bits of the group's earlier words (a kHRGroup-lane scan of popcounts)
+        for (int wb = (w_lo & ~63) + (tid & ~63); wb <= w_hi; wb += BLOCK) {
+          const int lane = tid & 63, w = wb + lane;
+          const unsigned full = w < W ? s_bits[w] : 0u;
+          const int pc = __popc(full);
+          int inc = pc;
+#pragma unroll
+          for (int d = 1; d < kHRGroup; d <<= 1) {
+            const int t = __shfl_up(inc, d, kHRGroup);
+            if ((lane & (kHRGroup - 1)) >= d) inc += t;
+          }
+          unsigned m = (w >= w_lo && w <= w_hi) ? full : 0u;
+          if (w == w_lo) m &= ~((1u << (c_lo & 31)) - 1u);
+          if (c_hi != INT_MAX && w == (c_hi >> 5)) m &= (1u << (c_hi & 31)) - 1u;
+          if (m) {
+            const int r0 = s_gpre[w / kHRGroup] + inc - pc - (int)s0;
+            while (m) {
+              const int b = __ffs(m) - 1;
+              s_colbuf[r0 + __popc(full & ((1u << b) - 1u))] = w * 32 + b;
+              m &= m - 1;
+            }
           }
         }
-      }
-      __syncthreads();
-      for (int t = tid; t < len; t += BLOCK) tile.c_col[base + s0 + t] = s_colbuf[t];
-      __syncthreads();
-      for (int t = tid; t < len; t += BLOCK) s_acc[t] = 0.0;
-      __syncthreads();
-      auto add = [&](int j, double v) {
-        if (whole || (j >= c_lo && j < c_hi)) atomicAdd(&s_acc[hr_rank(s_bits, s_gpre, j) - (int)s0], v);
-      };
-      if (one_chunk) {
-        RP st = my_st, ln = my_len;
-        if (!whole && tid < na) {  // the part of the entry's B row inside the slice's columns
-          const RP lo = s0 == 0 ? 0 : hr_lower_bound(tile.b_col, my_st, my_len, c_lo);
-          const RP hi = c_hi == INT_MAX ? my_len : hr_lower_bound(tile.b_col, my_st, my_len, c_hi);
-          st = my_st + lo;
-          ln = hi - lo;
+        __syncthreads();
+        if (s0 == 0) advance(5);
+        for (int t = tid; t < len; t += BLOCK) tile.c_col[base + s0 + t] = s_colbuf[t];
+        __syncthreads();
+        for (int t = tid; t < len; t += BLOCK) s_acc[t] = 0.0;
+        __syncthreads();
+        if (s0 == 0) advance(6);
+        auto add = [&](int j, double v) {
+          if (whole || (j >= c_lo && j < c_hi)) atomicAdd(&s_acc[hr_rank(s_bits, s_gpre, j) - (int)s0], v);
+        };
+        if (one_chunk) {
+          RP st = my_st, ln = my_len;
+          if (!whole && tid < na) {  // the part of the entry's B row inside the slice's columns
+            const RP lo = my_lo;
+            const RP hi = c_hi == INT_MAX ? my_len : hr_lower_bound(tile.b_col, my_st, lo, my_len, c_hi);
+            st = my_st + lo;
+            ln = hi - lo;
+            my_lo = hi;
+          }
+          run_staged(st, ln, add, std::true_type{});
+        } else {
+          for_each_product<BLOCK, BLOCK, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr, add);
+          __syncthreads();
         }
-        run_staged(st, ln, add, std::true_type{});
-      } else {
-        for_each_product<BLOCK, BLOCK, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr, add);
+        for (int t = tid; t < len; t += BLOCK) tile.c_val[base + s0 + t] = (T)s_acc[t];
         __syncthreads();
       }
-      for (int t = tid; t < len; t += BLOCK) tile.c_val[base + s0 + t] = (T)s_acc[t];
-      __syncthreads();
-    }
+    } while (false);
+    advance(6);  // rows that ended early (symbolic, refused) finish the next row's fetch here
   }
 }
 
