@@ -325,9 +325,9 @@ __device__ inline long long group_excl_scan(long long v, long long* total, long 
 // e_pre[e + 1] - e_pre[e] - 1 (e_pre holds ne + 1 prefix counts, e_pre[ne] = total). Lane l takes products l,
 // l + G, ..., U at a time: the U entry lookups (binary searches over the prefixes) advance in lockstep and the U loads
 // are issued together, so their latencies overlap. f(j, v) per product (v = a * b in float64; 0 when !NUM).
-template <int G, bool NUM, int U, typename T, typename RP, typename F>
+template <int G, bool NUM, int U, typename T, typename RP, typename PT, typename F>
 __device__ inline void enumerate_staged(const TileDev<T, RP>& tile, int ne, long long total, int lane,
-                                        const RP* e_start, const long long* e_pre, const T* e_val,
+                                        const RP* e_start, const PT* e_pre, const T* e_val,
                                         const volatile int* stop, F&& f) {
   int top = 1;  // largest power of two < ne (0 when ne == 1)
   while (top < ne) top <<= 1;
@@ -364,9 +364,9 @@ __device__ inline void enumerate_staged(const TileDev<T, RP>& tile, int ne, long
 // Load-balanced enumeration of the products of one row: the group stages the B-row ranges of up to G entries of
 // A's row in LDS, prefix-sums their lengths and runs enumerate_staged over them. `stop` (optional, LDS) ends the
 // enumeration early once set.
-template <int G, int BLOCK, bool NUM, typename T, typename RP, typename F, int U = 4>
+template <int G, int BLOCK, bool NUM, typename T, typename RP, typename PT, typename F, int U = 4>
 __device__ inline void for_each_product(const TileDev<T, RP>& tile, RP a0, RP a1, int lane, RP* e_start,
-                                        long long* e_pre, T* e_val, long long* scan_smem, const volatile int* stop,
+                                        PT* e_pre, T* e_val, long long* scan_smem, const volatile int* stop,
                                         F&& f) {
   for (RP c0 = a0; c0 < a1; c0 += G) {
     const int ne = (int)((a1 - c0) < (RP)G ? (a1 - c0) : (RP)G);
@@ -380,8 +380,8 @@ __device__ inline void for_each_product(const TileDev<T, RP>& tile, RP a0, RP a1
     }
     long long total;
     const long long ex = group_excl_scan<G, BLOCK>(len, &total, scan_smem);
-    if (lane < ne) e_pre[lane] = ex;
-    if (lane == 0) e_pre[ne] = total;
+    if (lane < ne) e_pre[lane] = (PT)ex;
+    if (lane == 0) e_pre[ne] = (PT)total;
     group_sync<G>();
     enumerate_staged<G, NUM, U>(tile, ne, total, lane, e_start, e_pre, e_val, stop, f);
     group_sync<G>();
@@ -1079,9 +1079,9 @@ constexpr int kHRWords = 16384;  // bitmap words: tiles up to 2^19 columns
 // 16-byte LDS load. Narrow groups cost prefix space (16 KB for 2^19 columns) but keep the per-product LDS reads at
 // 20 bytes; 16-word groups (four loads) were measured 0.75 ms slower on the s22 stage despite larger slices.
 constexpr int kHRGroup = 4;
-// float64 accumulators per slice: what LDS holds beside the bitmap, the prefixes and the entry staging (60 KB or 52 KB)
+// float64 accumulators per slice: what LDS holds beside the bitmap, the prefixes and the entry staging (64 KB or 56 KB)
 template <typename T, typename RP>
-constexpr int hr_slice() { return sizeof(T) == 4 && sizeof(RP) == 4 ? 7680 : 6656; }
+constexpr int hr_slice() { return sizeof(T) == 4 && sizeof(RP) == 4 ? 8192 : 7168; }
 // symbolic: 512 threads, two workgroups per CU; numeric: 1024 threads, one workgroup per CU (bitmap + slice)
 template <int MODE>
 constexpr int hr_block() { return MODE == 0 ? 512 : 1024; }
@@ -1174,7 +1174,8 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
   __shared__ long long s_base;
   __shared__ int s_crange[2];
   __shared__ RP s_est[BLOCK];
-  __shared__ long long s_epre[BLOCK + 1];
+  // product prefixes of one staged chunk fit 32 bits: at most BLOCK entries, each a B row of at most 2^19 columns
+  __shared__ int s_epre[BLOCK + 1];
   __shared__ T s_eval[BLOCK];
   __shared__ long long s_lscan[BLOCK / 64 + 1];
   constexpr int kGroupsPerThread = (kHRWords / kHRGroup + BLOCK - 1) / BLOCK;
@@ -1263,8 +1264,8 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
         }
         long long tot;
         const long long ex = block_excl_scan<BLOCK, long long>((long long)(tid < na ? len : 0), &tot, s_lscan);
-        if (tid < na) s_epre[tid] = ex;
-        if (tid == 0) s_epre[na] = tot;
+        if (tid < na) s_epre[tid] = (int)ex;
+        if (tid == 0) s_epre[na] = (int)tot;
         __syncthreads();
         enumerate_staged<BLOCK, kNum, 4>(tile, na, tot, tid, s_est, s_epre, s_eval, nullptr, f);
         __syncthreads();
