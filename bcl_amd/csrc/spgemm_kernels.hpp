@@ -1190,7 +1190,8 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
   // row then starts with its staging in registers instead of behind a chain of global round trips.
   int nq = 0;        // thread 0: the next row's queue slot
   int pst = 0;       // fetch steps done for the next row (uniform)
-  int n_idx = 0, n_fr = 0, n_tix = 0, n_kk = 0;
+  int n_idx = 0, n_fr = 0, n_tix = 0, n_kk = 0, n_cap = 0;  // n_cap, n_off: thread 0 only
+  long long n_off = 0;
   RP n_a0 = 0, n_a1 = 0, n_st = 0, n_len = 0;
   T n_av = T(0);
   auto advance = [&](int upto) {  // all threads, uniform upto; a barrier separates steps 1 and 2
@@ -1214,6 +1215,10 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
         const int64_t ni = n_fr - t.flat0;
         n_a0 = t.a_rp[ni];
         n_a1 = t.a_rp[ni + 1];
+        if (NUM && tid == 0) {  // only this block reads or rewrites the row's slot
+          n_cap = R.cap[n_fr];
+          n_off = R.off[n_fr];
+        }
       }
       pst = 4;
     }
@@ -1249,6 +1254,8 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
     const bool one_chunk = na <= BLOCK;  // one entry per thread, staged once and kept in registers
     const RP my_st = n_st, my_len = n_len;
     const T my_av = n_av;
+    const int my_cap = n_cap;
+    const long long my_off = n_off;
     if (tid == 0) nq = (int)atomicAdd(R.heavy_queue, 1ull);
     pst = 0;
     n_st = 0;
@@ -1316,8 +1323,8 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
       }
       if (tid == 0) {
         long long b;
-        if (MODE == 1 || (MODE == 2 && total <= (long long)R.cap[fr])) {
-          b = R.off[fr];
+        if (MODE == 1 || (MODE == 2 && total <= (long long)my_cap)) {
+          b = my_off;
         } else {
           b = (long long)atomicAdd(R.cursor, (unsigned long long)total);
           R.off[fr] = b;
