@@ -132,7 +132,8 @@ void FlatRows<T, RP>::reset_counters(hipStream_t s) {
 
 template <typename T, typename RP>
 void FlatRows<T, RP>::analyze(hipStream_t s, bool do_bin, long long pred_min, bool with_stats) {
-  hipLaunchKernelGGL((k_analyze<T, RP>), dim3((unsigned)nblocks), dim3(kRowBlock), 0, s, R, do_bin ? 1 : 0,
+  const unsigned grid = (unsigned)((nblocks + kAnaSub - 1) / kAnaSub);
+  hipLaunchKernelGGL((k_analyze<T, RP>), dim3(grid), dim3(kRowBlock), 0, s, R, (int64_t)nblocks, do_bin ? 1 : 0,
                      pred_min, with_stats ? 1 : 0);
   BCLX_HIP(hipGetLastError());
 }

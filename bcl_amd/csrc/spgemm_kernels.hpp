@@ -171,86 +171,147 @@ __device__ inline V block_sum(V v, V* smem) {
 // row analysis: products per row, optional binning by upper bound min(products, n) (symbolic bins)
 // ------------------------------------------------------------------------------------------------------------
 
+// 256-row blocks of flat rows per analysis workgroup: each thread analyzes one row of each, with the loads of its
+// rows issued together, and the workgroup takes its bin and list positions with one global atomic per bin.
+constexpr int kAnaSub = 4;
+
 template <typename T, typename RP>
-__global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int do_bin, long long pred_min,
-                                                       int with_stats) {
-  __shared__ int s_long[kRowBlock];
+__global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int64_t nblocks, int do_bin,
+                                                       long long pred_min, int with_stats) {
+  constexpr int kRows = kAnaSub * kRowBlock;
+  __shared__ int s_long[kRows];           // slots (sub * kRowBlock + tid) of rows with more than 32 entries
+  __shared__ long long s_lprod[kRows];    // their products, by slot
   __shared__ int s_nlong;
   __shared__ int s_bcount[kNumBins], s_bbase[kNumBins];
   __shared__ int s_npred, s_pbase;
   __shared__ long long s_red[kRowBlock / 64 + 1];
-  const int tid = threadIdx.x;
-  const int64_t fr = (int64_t)blockIdx.x * kRowBlock + tid;
-  const TileDev<T, RP> tile = R.tiles[R.blk_tile[blockIdx.x]];
-  const int64_t i = fr - tile.flat0;
-  const bool valid = i >= 0 && i < tile.m;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t sb0 = (int64_t)blockIdx.x * kAnaSub;
   if (tid == 0) {
     s_nlong = 0;
     s_npred = 0;
   }
   if (tid < kNumBins) s_bcount[tid] = 0;
   __syncthreads();
-  long long prod = 0;
-  if (valid) {
-    RP a0 = tile.a_rp[i], a1 = tile.a_rp[i + 1];
-    if (a1 - a0 <= 32) {
-      // eight entries at a time: their column loads, then their B-row-pointer loads, are issued together
-      for (RP p0 = a0; p0 < a1; p0 += 8) {
-        int k[8];
+  const TileDev<T, RP>* tp[kAnaSub];
+  bool in[kAnaSub], valid[kAnaSub];
+  int64_t i[kAnaSub];
+  RP a0[kAnaSub], a1[kAnaSub];
+  long long prod[kAnaSub];
+  bool lng[kAnaSub];
 #pragma unroll
-        for (int u = 0; u < 8; u++) k[u] = p0 + u < a1 ? tile.a_col[p0 + u] : -1;
+  for (int s = 0; s < kAnaSub; s++) {
+    in[s] = sb0 + s < nblocks;
+    tp[s] = R.tiles + (in[s] ? R.blk_tile[sb0 + s] : 0);
+    i[s] = (sb0 + s) * kRowBlock + tid - tp[s]->flat0;
+    valid[s] = in[s] && i[s] >= 0 && i[s] < tp[s]->m;
+    a0[s] = valid[s] ? tp[s]->a_rp[i[s]] : 0;
+    a1[s] = valid[s] ? tp[s]->a_rp[i[s] + 1] : 0;
+    prod[s] = 0;
+  }
+  // rows of up to 32 entries: four entries of every row at a time, their column loads and then their B-row-pointer
+  // loads issued together; longer rows are listed for the wavefronts
+  bool more = false;
 #pragma unroll
-        for (int u = 0; u < 8; u++)
-          if (k[u] >= 0) prod += (long long)(tile.b_rp[k[u] + 1] - tile.b_rp[k[u]]);
-      }
-    } else {
-      s_long[atomicAdd(&s_nlong, 1)] = tid;
+  for (int s = 0; s < kAnaSub; s++) {
+    lng[s] = a1[s] - a0[s] > 32;
+    if (lng[s]) {
+      s_long[atomicAdd(&s_nlong, 1)] = s * kRowBlock + tid;
+      a1[s] = a0[s];
+    }
+    more = more || a1[s] > a0[s];
+  }
+  for (RP p0 = 0; more; p0 += 4) {
+    int k[kAnaSub][4];
+    more = false;
+#pragma unroll
+    for (int s = 0; s < kAnaSub; s++)
+#pragma unroll
+      for (int u = 0; u < 4; u++) k[s][u] = a0[s] + p0 + u < a1[s] ? tp[s]->a_col[a0[s] + p0 + u] : -1;
+#pragma unroll
+    for (int s = 0; s < kAnaSub; s++) {
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (k[s][u] >= 0) prod[s] += (long long)(tp[s]->b_rp[k[s][u] + 1] - tp[s]->b_rp[k[s][u]]);
+      more = more || a0[s] + p0 + 4 < a1[s];
     }
   }
   __syncthreads();
+  // rows of more than 32 entries: one wavefront each, lanes striding over the entries
   const int nlong = s_nlong;
-  for (int q = 0; q < nlong; q++) {
-    const int lt = s_long[q];
-    const int64_t li = (int64_t)blockIdx.x * kRowBlock + lt - tile.flat0;
-    RP a0 = tile.a_rp[li], a1 = tile.a_rp[li + 1];
-    long long s = 0;
-    for (RP p = a0 + tid; p < a1; p += kRowBlock) {
-      int k = tile.a_col[p];
-      s += (long long)(tile.b_rp[k + 1] - tile.b_rp[k]);
+  for (int q = wave; q < nlong; q += kRowBlock / 64) {
+    const int slot = s_long[q], s = slot / kRowBlock;
+    const TileDev<T, RP>* t = R.tiles + R.blk_tile[sb0 + s];
+    const int64_t li = (sb0 + s) * kRowBlock + (slot % kRowBlock) - t->flat0;
+    const RP b0 = t->a_rp[li], b1 = t->a_rp[li + 1];
+    long long sum = 0;
+    for (RP p = b0 + lane; p < b1; p += 8 * 64) {  // eight entries per lane in flight
+      int kk[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) kk[u] = p + 64 * u < b1 ? t->a_col[p + 64 * u] : -1;
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (kk[u] >= 0) sum += (long long)(t->b_rp[kk[u] + 1] - t->b_rp[kk[u]]);
     }
-    s = block_sum<kRowBlock, long long>(s, s_red);
-    if (tid == lt) prod = s;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d);
+    if (lane == 0) s_lprod[slot] = sum;
   }
-  R.prod[fr] = valid ? prod : 0;
-  const long long ub0 = prod < (long long)tile.n ? prod : (long long)tile.n;
-  const int b0 = valid ? bin_for(ub0, prod, R.tn, tile.n <= (1 << kEscColBits)) : 0;
-  if (with_stats && b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
-  bool pr = false;
+  __syncthreads();
+  int b0[kAnaSub];
+  bool pr[kAnaSub];
+  long long ub0[kAnaSub];
+#pragma unroll
+  for (int s = 0; s < kAnaSub; s++) {
+    if (lng[s]) prod[s] = s_lprod[s * kRowBlock + tid];
+    const int64_t fr = (sb0 + s) * kRowBlock + tid;
+    if (in[s]) R.prod[fr] = valid[s] ? prod[s] : 0;
+    ub0[s] = prod[s] < (long long)tp[s]->n ? prod[s] : (long long)tp[s]->n;
+    b0[s] = valid[s] ? bin_for(ub0[s], prod[s], R.tn, tp[s]->n <= (1 << kEscColBits)) : 0;
+    if (with_stats && b0[s] == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
+    pr[s] = false;
+  }
   if (pred_min >= 0) {
     // learned path: every row starts at the exact bound min(prod, n); rows with more than pred_min products that
     // hash (sorted rows are exact in one pass and keep the bound) go on the list whose capacities the predictor
     // then lowers, and are binned after the prediction
-    R.cap[fr] = valid ? (int32_t)ub0 : 0;
-    pr = valid && prod > pred_min && b0 < kEscBin0;
-    int ppos = 0;
-    if (pr) ppos = atomicAdd(&s_npred, 1);
+    int ppos[kAnaSub];
+#pragma unroll
+    for (int s = 0; s < kAnaSub; s++) {
+      const int64_t fr = (sb0 + s) * kRowBlock + tid;
+      if (in[s]) R.cap[fr] = valid[s] ? (int32_t)ub0[s] : 0;
+      pr[s] = valid[s] && prod[s] > pred_min && b0[s] < kEscBin0;
+      ppos[s] = pr[s] ? atomicAdd(&s_npred, 1) : 0;
+    }
     __syncthreads();
     if (tid == 0 && s_npred > 0) s_pbase = atomicAdd(R.pred_count, s_npred);
     __syncthreads();
-    if (pr) R.pred_items[s_pbase + ppos] = (int32_t)fr;
+#pragma unroll
+    for (int s = 0; s < kAnaSub; s++)
+      if (pr[s]) R.pred_items[s_pbase + ppos[s]] = (int32_t)((sb0 + s) * kRowBlock + tid);
   }
   if (do_bin) {
-    const int b = pr ? 0 : b0;
-    int pos = 0;
-    if (b > 0) pos = atomicAdd(&s_bcount[b], 1);
+    int b[kAnaSub], pos[kAnaSub];
+#pragma unroll
+    for (int s = 0; s < kAnaSub; s++) {
+      b[s] = pr[s] ? 0 : b0[s];
+      pos[s] = b[s] > 0 ? atomicAdd(&s_bcount[b[s]], 1) : 0;
+    }
     __syncthreads();
     if (tid > 0 && tid < kNumBins && s_bcount[tid] > 0) s_bbase[tid] = atomicAdd(R.count_of(tid), s_bcount[tid]);
     __syncthreads();
-    if (b > 0) R.bin_items[b * R.bin_stride + s_bbase[b] + pos] = (int)fr;
-    if (b == 0 && !pr) R.nnz[fr] = 0;  // empty and padding rows
+#pragma unroll
+    for (int s = 0; s < kAnaSub; s++) {
+      const int64_t fr = (sb0 + s) * kRowBlock + tid;
+      if (b[s] > 0) R.bin_items[b[s] * R.bin_stride + s_bbase[b[s]] + pos[s]] = (int)fr;
+      if (in[s] && b[s] == 0 && !pr[s]) R.nnz[fr] = 0;  // empty and padding rows
+    }
   }
   if (with_stats) {  // one global atomic per block: only when asked (a plan counts products once)
-    long long tot = block_sum<kRowBlock, long long>(prod, s_red);
+    long long mine = 0;
+#pragma unroll
+    for (int s = 0; s < kAnaSub; s++) mine += prod[s];
+    long long tot = block_sum<kRowBlock, long long>(mine, s_red);
     if (tid == 0 && tot) atomicAdd(&R.stats[0], (unsigned long long)tot);
   }
 }
@@ -1536,14 +1597,27 @@ constexpr int kScanBlock = 1024;
 constexpr int kScanPer = 4;
 constexpr int kScanTile = kScanBlock * kScanPer;
 
+// kScanPer consecutive counts of a thread: one 16-byte load inside the array (cnt is 16-byte aligned), else scalars
+__device__ inline void scan_load4(const int32_t* cnt, int64_t n, int64_t r0, long long* v) {
+  static_assert(kScanPer == 4, "one int4 per thread");
+  if (r0 + 3 < n) {
+    const int4 x = *reinterpret_cast<const int4*>(cnt + r0);
+    v[0] = x.x;
+    v[1] = x.y;
+    v[2] = x.z;
+    v[3] = x.w;
+  } else {
+#pragma unroll
+    for (int q = 0; q < kScanPer; q++) v[q] = r0 + q < n ? cnt[r0 + q] : 0;
+  }
+}
+
 static __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int32_t* cnt, int64_t n, long long* partial) {
   __shared__ long long sm[kScanBlock / 64 + 1];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile;
-  long long s = 0;
-  for (int q = 0; q < kScanPer; q++) {
-    const int64_t r = base + (int64_t)q * kScanBlock + threadIdx.x;
-    if (r < n) s += cnt[r];
-  }
+  const int64_t r0 = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+  long long v[kScanPer];
+  scan_load4(cnt, n, r0, v);
+  long long s = v[0] + v[1] + v[2] + v[3];
   s = block_sum<kScanBlock, long long>(s, sm);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
@@ -1569,17 +1643,22 @@ static __global__ __launch_bounds__(kScanBlock) void k_scan_final(const int32_t*
   // each thread owns kScanPer consecutive elements
   const int64_t r0 = base + (int64_t)threadIdx.x * kScanPer;
   long long v[kScanPer];
-  long long s = 0;
-  for (int q = 0; q < kScanPer; q++) {
-    v[q] = (r0 + q < n) ? cnt[r0 + q] : 0;
-    s += v[q];
-  }
+  scan_load4(cnt, n, r0, v);
+  const long long s = v[0] + v[1] + v[2] + v[3];
   long long tot;
   long long ex = block_excl_scan<kScanBlock, long long>(s, &tot, sm);
   long long run = partial[blockIdx.x] + ex + off_add;
-  for (int q = 0; q < kScanPer; q++) {
-    if (r0 + q < n) off[r0 + q] = run;
-    run += v[q];
+  if (r0 + 3 < n) {  // two 16-byte stores (off is 16-byte aligned and r0 a multiple of 4)
+    typedef long long ll2 __attribute__((ext_vector_type(2)));
+    ll2* o = reinterpret_cast<ll2*>(off + r0);
+    o[0] = ll2{run, run + v[0]};
+    o[1] = ll2{run + v[0] + v[1], run + v[0] + v[1] + v[2]};
+    run += s;
+  } else {
+    for (int q = 0; q < kScanPer; q++) {
+      if (r0 + q < n) off[r0 + q] = run;
+      run += v[q];
+    }
   }
   if (r0 + kScanPer >= n && r0 < n) off[n] = run;  // the thread holding the last element writes the total
 }
