@@ -81,7 +81,7 @@ void FlatRows<T, RP>::setup(bclx_handle* h_, const std::vector<TileDev<T, RP>>& 
   size_t o_off = A.reserve(sizeof(int64_t) * (nflat + 1));
   size_t o_bc = A.reserve(sizeof(int32_t) * kNumBins * kCountStride);
   size_t o_bi = A.reserve(sizeof(int32_t) * kNumBins * nflat);
-  size_t o_oc = A.reserve(sizeof(int32_t) * 2);
+  size_t o_oc = A.reserve(sizeof(int32_t) * 3);
   size_t o_oi = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
   size_t o_pi = A.reserve(sizeof(int32_t) * (with_cap ? nflat : 1));
   size_t o_cur = A.reserve(sizeof(unsigned long long) * 2);
@@ -103,6 +103,7 @@ void FlatRows<T, RP>::setup(bclx_handle* h_, const std::vector<TileDev<T, RP>>& 
   R.ovf_count = A.at<int32_t>(o_oc);
   R.ovf_items = A.at<int32_t>(o_oi);
   R.pred_count = R.ovf_count + 1;
+  R.long_count = R.ovf_count + 2;
   R.pred_items = A.at<int32_t>(o_pi);
   R.cursor = A.at<unsigned long long>(o_cur);
   R.heavy_queue = R.cursor + 1;
@@ -126,15 +127,19 @@ void FlatRows<T, RP>::upload_tiles() {
 template <typename T, typename RP>
 void FlatRows<T, RP>::reset_counters(hipStream_t s) {
   fill_i32(s, R.bin_count, kNumBins * kCountStride, 0);
-  fill_i32(s, R.ovf_count, 2, 0);
+  fill_i32(s, R.ovf_count, 3, 0);
   fill_u64(s, R.stats, 8, 0);
 }
 
 template <typename T, typename RP>
 void FlatRows<T, RP>::analyze(hipStream_t s, bool do_bin, long long pred_min, bool with_stats) {
   const unsigned grid = (unsigned)((nblocks + kAnaSub - 1) / kAnaSub);
+  fill_i32(s, R.long_count, 1, 0);
   hipLaunchKernelGGL((k_analyze<T, RP>), dim3(grid), dim3(kRowBlock), 0, s, R, (int64_t)nblocks, do_bin ? 1 : 0,
                      pred_min, with_stats ? 1 : 0);
+  BCLX_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_analyze_long<T, RP>), dim3((unsigned)h->num_cus * 8), dim3(kRowBlock), 0, s, R,
+                     do_bin ? 1 : 0, pred_min, with_stats ? 1 : 0);
   BCLX_HIP(hipGetLastError());
 }
 

@@ -104,6 +104,7 @@ struct RowsDev {
   int32_t* ovf_count;        // overflow list (learned path)
   int32_t* ovf_items;
   int32_t* pred_count;       // rows whose capacity the predictor sets (learned path)
+  int32_t* long_count;       // rows of more than 32 A entries, listed by the analysis in tmp_items
   int32_t* pred_items;
   unsigned long long* cursor;  // bump allocator for rows placed outside their predicted slot
   unsigned long long* heavy_queue;  // next heavy row to take (rows are handed out largest first)
@@ -179,13 +180,11 @@ template <typename T, typename RP>
 __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int64_t nblocks, int do_bin,
                                                        long long pred_min, int with_stats) {
   constexpr int kRows = kAnaSub * kRowBlock;
-  __shared__ int s_long[kRows];           // slots (sub * kRowBlock + tid) of rows with more than 32 entries
-  __shared__ long long s_lprod[kRows];    // their products, by slot
-  __shared__ int s_nlong;
+  __shared__ int s_nlong, s_lbase;
   __shared__ int s_bcount[kNumBins], s_bbase[kNumBins];
   __shared__ int s_npred, s_pbase;
   __shared__ long long s_red[kRowBlock / 64 + 1];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x;
   const int64_t sb0 = (int64_t)blockIdx.x * kAnaSub;
   if (tid == 0) {
     s_nlong = 0;
@@ -210,15 +209,15 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int64_t
     prod[s] = 0;
   }
   // rows of up to 32 entries: four entries of every row at a time, their column loads and then their B-row-pointer
-  // loads issued together; longer rows are listed for the wavefronts
+  // loads issued together. Longer rows (R-MAT hubs cluster in a few workgroups) go on a global list that
+  // k_analyze_long spreads over the whole GPU, so they do not hold their workgroup.
   bool more = false;
+  int lpos[kAnaSub];
 #pragma unroll
   for (int s = 0; s < kAnaSub; s++) {
     lng[s] = a1[s] - a0[s] > 32;
-    if (lng[s]) {
-      s_long[atomicAdd(&s_nlong, 1)] = s * kRowBlock + tid;
-      a1[s] = a0[s];
-    }
+    lpos[s] = lng[s] ? atomicAdd(&s_nlong, 1) : 0;
+    if (lng[s]) a1[s] = a0[s];
     more = more || a1[s] > a0[s];
   }
   for (RP p0 = 0; more; p0 += 4) {
@@ -237,34 +236,19 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int64_t
     }
   }
   __syncthreads();
-  // rows of more than 32 entries: one wavefront each, lanes striding over the entries
-  const int nlong = s_nlong;
-  for (int q = wave; q < nlong; q += kRowBlock / 64) {
-    const int slot = s_long[q], s = slot / kRowBlock;
-    const TileDev<T, RP>* t = R.tiles + R.blk_tile[sb0 + s];
-    const int64_t li = (sb0 + s) * kRowBlock + (slot % kRowBlock) - t->flat0;
-    const RP b0 = t->a_rp[li], b1 = t->a_rp[li + 1];
-    long long sum = 0;
-    for (RP p = b0 + lane; p < b1; p += 8 * 64) {  // eight entries per lane in flight
-      int kk[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) kk[u] = p + 64 * u < b1 ? t->a_col[p + 64 * u] : -1;
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (kk[u] >= 0) sum += (long long)(t->b_rp[kk[u] + 1] - t->b_rp[kk[u]]);
-    }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d);
-    if (lane == 0) s_lprod[slot] = sum;
-  }
+  if (tid == 0 && s_nlong > 0) s_lbase = atomicAdd(R.long_count, s_nlong);
   __syncthreads();
   int b0[kAnaSub];
   bool pr[kAnaSub];
   long long ub0[kAnaSub];
 #pragma unroll
   for (int s = 0; s < kAnaSub; s++) {
-    if (lng[s]) prod[s] = s_lprod[s * kRowBlock + tid];
     const int64_t fr = (sb0 + s) * kRowBlock + tid;
+    if (lng[s]) {  // finished by k_analyze_long
+      R.tmp_items[s_lbase + lpos[s]] = (int32_t)fr;
+      valid[s] = false;
+      in[s] = false;
+    }
     if (in[s]) R.prod[fr] = valid[s] ? prod[s] : 0;
     ub0[s] = prod[s] < (long long)tp[s]->n ? prod[s] : (long long)tp[s]->n;
     b0[s] = valid[s] ? bin_for(ub0[s], prod[s], R.tn, tp[s]->n <= (1 << kEscColBits)) : 0;
@@ -313,6 +297,52 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int64_t
     for (int s = 0; s < kAnaSub; s++) mine += prod[s];
     long long tot = block_sum<kRowBlock, long long>(mine, s_red);
     if (tid == 0 && tot) atomicAdd(&R.stats[0], (unsigned long long)tot);
+  }
+}
+
+// The rows k_analyze listed (more than 32 A entries): one wavefront per row, eight entries per lane in flight, then
+// the same outputs as k_analyze gives a row (products, capacity, predictor list, bin), with per-row atomics.
+template <typename T, typename RP>
+__global__ __launch_bounds__(kRowBlock) void k_analyze_long(RowsDev<T, RP> R, int do_bin, long long pred_min,
+                                                            int with_stats) {
+  const int lane = threadIdx.x & 63;
+  const int nwaves = (int)gridDim.x * (kRowBlock / 64);
+  const int cnt = *R.long_count;
+  for (int q = (int)blockIdx.x * (kRowBlock / 64) + (threadIdx.x >> 6); q < cnt; q += nwaves) {
+    const int fr = R.tmp_items[q];
+    const TileDev<T, RP>* t = R.tiles + R.blk_tile[fr >> 8];
+    const int64_t i = fr - t->flat0;
+    const RP a0 = t->a_rp[i], a1 = t->a_rp[i + 1];
+    long long prod = 0;
+    for (RP p = a0 + lane; p < a1; p += 8 * 64) {
+      int kk[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) kk[u] = p + 64 * u < a1 ? t->a_col[p + 64 * u] : -1;
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (kk[u] >= 0) prod += (long long)(t->b_rp[kk[u] + 1] - t->b_rp[kk[u]]);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) prod += __shfl_xor(prod, d);
+    if (lane != 0) continue;
+    R.prod[fr] = prod;
+    const long long ub0 = prod < (long long)t->n ? prod : (long long)t->n;
+    const int b0 = bin_for(ub0, prod, R.tn, t->n <= (1 << kEscColBits));
+    if (with_stats) {
+      atomicAdd(&R.stats[0], (unsigned long long)prod);
+      if (b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
+    }
+    bool pr = false;
+    if (pred_min >= 0) {
+      R.cap[fr] = (int32_t)ub0;
+      pr = prod > pred_min && b0 < kEscBin0;
+      if (pr) R.pred_items[atomicAdd(R.pred_count, 1)] = fr;
+    }
+    if (do_bin) {
+      const int b = pr ? 0 : b0;
+      if (b > 0) R.bin_items[b * R.bin_stride + atomicAdd(R.count_of(b), 1)] = fr;
+      if (b == 0 && !pr) R.nnz[fr] = 0;
+    }
   }
 }
 
