@@ -300,15 +300,20 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int64_t
   }
 }
 
-// The rows k_analyze listed (more than 32 A entries): one wavefront per row, eight entries per lane in flight, then
-// the same outputs as k_analyze gives a row (products, capacity, predictor list, bin), with per-row atomics.
+// The rows k_analyze listed (more than 32 A entries): one wavefront per row, rows strided over the wavefronts (the
+// list holds each workgroup's long rows together, so striding spreads neighbouring hubs), eight entries per lane in
+// flight, then the same outputs as k_analyze gives a row (products, capacity, predictor list, bin), with per-row
+// atomics.
+// The loop control is wave-uniform arithmetic and lane 0's writes are one if-block at the end of the body: no other
+// lane-0 region in the loop (a queue pop by lane 0 at the top was merged by the compiler with the writes at the
+// bottom, and the other lanes then looped on their own on a stale index).
 template <typename T, typename RP>
 __global__ __launch_bounds__(kRowBlock) void k_analyze_long(RowsDev<T, RP> R, int do_bin, long long pred_min,
                                                             int with_stats) {
   const int lane = threadIdx.x & 63;
   const int nwaves = (int)gridDim.x * (kRowBlock / 64);
-  const int cnt = *R.long_count;
-  for (int q = (int)blockIdx.x * (kRowBlock / 64) + (threadIdx.x >> 6); q < cnt; q += nwaves) {
+  const int cnt = R.long_count[0];
+  for (int q = (int)blockIdx.x * (kRowBlock / 64) + (int)(threadIdx.x >> 6); q < cnt; q += nwaves) {
     const int fr = R.tmp_items[q];
     const TileDev<T, RP>* t = R.tiles + R.blk_tile[fr >> 8];
     const int64_t i = fr - t->flat0;
@@ -324,24 +329,25 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze_long(RowsDev<T, RP> R, in
     }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) prod += __shfl_xor(prod, d);
-    if (lane != 0) continue;
-    R.prod[fr] = prod;
-    const long long ub0 = prod < (long long)t->n ? prod : (long long)t->n;
-    const int b0 = bin_for(ub0, prod, R.tn, t->n <= (1 << kEscColBits));
-    if (with_stats) {
-      atomicAdd(&R.stats[0], (unsigned long long)prod);
-      if (b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
-    }
-    bool pr = false;
-    if (pred_min >= 0) {
-      R.cap[fr] = (int32_t)ub0;
-      pr = prod > pred_min && b0 < kEscBin0;
-      if (pr) R.pred_items[atomicAdd(R.pred_count, 1)] = fr;
-    }
-    if (do_bin) {
-      const int b = pr ? 0 : b0;
-      if (b > 0) R.bin_items[b * R.bin_stride + atomicAdd(R.count_of(b), 1)] = fr;
-      if (b == 0 && !pr) R.nnz[fr] = 0;
+    if (lane == 0) {
+      R.prod[fr] = prod;
+      const long long ub0 = prod < (long long)t->n ? prod : (long long)t->n;
+      const int b0 = bin_for(ub0, prod, R.tn, t->n <= (1 << kEscColBits));
+      if (with_stats) {
+        atomicAdd(&R.stats[0], (unsigned long long)prod);
+        if (b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
+      }
+      bool pr = false;
+      if (pred_min >= 0) {
+        R.cap[fr] = (int32_t)ub0;
+        pr = prod > pred_min && b0 < kEscBin0;
+        if (pr) R.pred_items[atomicAdd(R.pred_count, 1)] = fr;
+      }
+      if (do_bin) {
+        const int b = pr ? 0 : b0;
+        if (b > 0) R.bin_items[b * R.bin_stride + atomicAdd(R.count_of(b), 1)] = fr;
+        if (b == 0 && !pr) R.nnz[fr] = 0;
+      }
     }
   }
 }
