@@ -91,9 +91,10 @@ def test_stage_modes(handle, stage14, mode, pkind, graph):
 
 
 @pytest.mark.parametrize("tuning", [(64, 4096, 1, 32, 0), (16, 1024, 2, 0, 4096), (256, 2048, 1, 64, 256),
-                                    (64, 4096, 1, 32, 1024)])
+                                    (64, 4096, 1, 32, 1024), (16, 2048, 1, 16, 4096, 2), (64, 4096, 1, 32, 4096, 4)])
 def test_stage_tuning_choices(handle, stage14, tuning):
-    """Every binning choice (sorting off, all sortable rows sorted, narrow groups) gives the oracle's result."""
+    """Every binning choice (sorting off, all sortable rows sorted, narrow groups, bins run as two or four parallel
+    graph branches) gives the oracle's result."""
     print(f"[stage] tuning={tuning}", flush=True)
     A, B, pairs = stage_tiles(stage14, stage14, 2, 1)
     for mode in (BATCHED_EXACT, LEARNED):
