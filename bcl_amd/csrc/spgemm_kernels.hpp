@@ -179,7 +179,6 @@ constexpr int kAnaSub = 4;
 template <typename T, typename RP>
 __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int64_t nblocks, int do_bin,
                                                        long long pred_min, int with_stats) {
-  constexpr int kRows = kAnaSub * kRowBlock;
   __shared__ int s_nlong, s_lbase;
   __shared__ int s_bcount[kNumBins], s_bbase[kNumBins];
   __shared__ int s_npred, s_pbase;
