@@ -29,6 +29,16 @@ SPACE = {"bin_streams": [1, 2, 4], "esc_max": [0, 512, 1024, 2048, 4096], "esc_m
 KNOBS = ("work_per_lane", "heavy_need", "load_shift", "esc_min", "esc_max", "bin_streams")
 
 
+def load_tuning(path: str | None = None):
+    """The shipped stage settings: (h_tuning array, margin, slack), or (None, defaults) when no table is saved."""
+    path = path or os.path.join(DATA_DIR, "autotune.json")
+    if not os.path.exists(path):
+        return None, DEFAULT["margin"], DEFAULT["slack"]
+    with open(path) as f:
+        tj = json.load(f)
+    return np.array(tj["tuning"], np.int32), float(tj.get("margin", DEFAULT["margin"])), int(tj.get("slack", DEFAULT["slack"]))
+
+
 def time_plan(h, A, B, params, cfg, reps=3):
     import torch
     from .stage import LEARNED, StagePlan

@@ -4,6 +4,25 @@
 namespace bclx {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+thread_local AllocFns g_alloc;
+
+void* dev_alloc(size_t bytes, const AllocFns& how) {
+  void* p = nullptr;
+  if (how.alloc) {
+    p = how.alloc(bytes, how.user);
+    BCLX_REQUIRE(p != nullptr, BCLX_ERR_ALLOC,
+                 "the handle's allocator returned NULL for " + std::to_string(bytes) + " bytes");
+  } else {
+    BCLX_HIP(hipMalloc(&p, bytes));
+  }
+  return p;
+}
+
+void dev_free(void* p, const AllocFns& how) {
+  if (how.free) how.free(p, how.user);
+  else (void)hipFree(p);
+}
 }  // namespace bclx
 
 using namespace bclx;
@@ -40,6 +59,15 @@ bclx_status bclx_destroy(bclx_handle* h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     delete h;
+  });
+}
+
+bclx_status bclx_set_allocator(bclx_handle* h, bclx_alloc_fn alloc, bclx_free_fn free_fn, void* user) {
+  return guarded([&] {
+    BCLX_REQUIRE(h != nullptr, BCLX_ERR_INVALID, "null handle");
+    BCLX_REQUIRE((alloc == nullptr) == (free_fn == nullptr), BCLX_ERR_INVALID,
+                 "give both allocator functions, or neither (hipMalloc / hipFree)");
+    h->alloc = AllocFns{alloc, free_fn, user};
   });
 }
 

@@ -34,30 +34,45 @@ struct Error {
     if (!(cond)) throw ::bclx::Error{(status), (msg)};        \
   } while (0)
 
-// Device buffer owned by the library (hipMalloc / hipFree). Move-only.
+// The device allocator of the calling thread's current handle (bclx_set_allocator; hipMalloc / hipFree when unset).
+// Every entry point selects its handle's allocator through set_device() before it allocates.
+struct AllocFns {
+  bclx_alloc_fn alloc = nullptr;
+  bclx_free_fn free = nullptr;
+  void* user = nullptr;
+};
+extern thread_local AllocFns g_alloc;
+
+// Device memory from the current allocator; freed with the allocator it came from.
+void* dev_alloc(size_t bytes, const AllocFns& how);
+void dev_free(void* p, const AllocFns& how);
+
+// Device buffer owned by the library. Move-only; remembers the allocator that made it.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  AllocFns how;
   DevBuf() = default;
   explicit DevBuf(size_t n) { alloc(n); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), how(o.how) { o.p = nullptr; o.bytes = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; how = o.how; o.p = nullptr; o.bytes = 0; }
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t n) {
     release();
     if (n == 0) n = 256;
-    BCLX_HIP(hipMalloc(&p, n));
+    how = g_alloc;
+    p = dev_alloc(n, how);
     bytes = n;
   }
   // grow-only
   void reserve(size_t n) { if (n > bytes) alloc(n); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) dev_free(p, how);
     p = nullptr;
     bytes = 0;
   }
@@ -82,6 +97,7 @@ struct Arena {
 
 struct bclx_handle {
   int device = 0;
+  bclx::AllocFns alloc;  // device allocator of this handle (bclx_set_allocator)
   hipStream_t stream = nullptr;
   int num_cus = 256;
   size_t lds_per_block = 65536;
@@ -114,7 +130,10 @@ bclx_status guarded(F&& f) {
   }
 }
 
-inline void set_device(bclx_handle* h) { BCLX_HIP(hipSetDevice(h->device)); }
+inline void set_device(bclx_handle* h) {
+  BCLX_HIP(hipSetDevice(h->device));
+  g_alloc = h->alloc;
+}
 
 inline unsigned div_up(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 

@@ -271,26 +271,23 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
     return;
   }
   if (b == kHeavyBin) {  // global dense accumulator (tiles too wide for a window index)
+    // prepare_heavy() made the accumulators and bitmaps for every resident workgroup before any graph capture;
+    // nothing is allocated or attributed here
     const int64_t words = (max_n + 31) / 32;
-    const bool lds_bitmap = words * 4 <= (int64_t)h->lds_per_block - 4096;
+    const bool lds_bitmap = heavy_lds_bitmap();
+    BCLX_REQUIRE(h->dense_nslots >= h->num_cus && h->dense_slot_doubles >= (size_t)max_n &&
+                     (lds_bitmap || gbits.bytes >= sizeof(unsigned) * (size_t)words * h->num_cus),
+                 BCLX_ERR_INVALID, "heavy-row accumulators were not prepared");
     int nslots = (int)std::min<int64_t>(count_hint >= 0 ? count_hint : h->num_cus, (int64_t)h->num_cus);
     if (nslots < 1) nslots = 1;
-    ensure_dense_slots(h, nslots, (size_t)max_n);
-    if (!lds_bitmap) gbits.reserve(sizeof(unsigned) * (size_t)words * nslots + 256);
-    // zeroed global bitmaps are kept zero by the kernel; zero them once on (re)allocation
-    if (!lds_bitmap && !gbits_zeroed) {
-      fill_i32(s, gbits.as<int32_t>(), (int64_t)(gbits.bytes / 4), 0);
-      gbits_zeroed = true;
-    }
     const size_t shm = lds_bitmap ? (size_t)words * 4 : 0;
     double* dense = h->dense_slots.as<double>();
     const int64_t sd = (int64_t)h->dense_slot_doubles;
     unsigned* gb = lds_bitmap ? nullptr : gbits.as<unsigned>();
 #define BCLX_HEAVY(M, LB)                                                                                 \
   do {                                                                                                    \
-    auto kfn = k_heavy<M, LB, T, RP>;                                                                     \
-    if (shm > 0) BCLX_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)); \
-    hipLaunchKernelGGL(kfn, dim3(nslots), dim3(kHeavyThreads), shm, s, R, kHeavyBin, dense, sd, gb, words);  \
+    hipLaunchKernelGGL((k_heavy<M, LB, T, RP>), dim3(nslots), dim3(kHeavyThreads), shm, s, R, kHeavyBin, dense, sd, \
+                       gb, words);                                                                        \
   } while (0)
     if (lds_bitmap) {
       switch (mode) { case 0: BCLX_HEAVY(0, true); break; case 1: BCLX_HEAVY(1, true); break;
@@ -347,10 +344,32 @@ int64_t FlatRows<T, RP>::read_stat(int q) {
 }
 
 template <typename T, typename RP>
+bool FlatRows<T, RP>::heavy_lds_bitmap() const {
+  return ((max_n + 31) / 32) * 4 <= (int64_t)h->lds_per_block - 4096;
+}
+
+template <typename T, typename RP>
 void FlatRows<T, RP>::prepare_heavy() {
-  // the rank-slice kernel keeps everything in LDS; the global dense fallback (tiles wider than 2^19 columns) needs
-  // one accumulator of max_n doubles per resident workgroup
-  if ((max_n + 31) / 32 > kHRWords) ensure_dense_slots(h, h->num_cus, (size_t)max_n);
+  // The rank-slice kernel keeps everything in LDS. The global dense fallback (tiles wider than 2^19 columns) needs
+  // one accumulator of max_n doubles per resident workgroup, and a zeroed bitmap per workgroup when the tile is too
+  // wide for an LDS bitmap; the kernel leaves both zeroed behind every row.
+  if ((max_n + 31) / 32 <= kHRWords) return;
+  ensure_dense_slots(h, h->num_cus, (size_t)max_n);
+  const int64_t words = (max_n + 31) / 32;
+  if (heavy_lds_bitmap()) {
+    const int shm = (int)(words * 4);
+    BCLX_HIP(hipFuncSetAttribute((const void*)k_heavy<0, true, T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    BCLX_HIP(hipFuncSetAttribute((const void*)k_heavy<1, true, T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    BCLX_HIP(hipFuncSetAttribute((const void*)k_heavy<2, true, T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    BCLX_HIP(hipFuncSetAttribute((const void*)k_heavy<3, true, T, RP>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    return;
+  }
+  const size_t need = sizeof(unsigned) * (size_t)words * (size_t)h->num_cus;
+  if (gbits.bytes < need) {
+    gbits.alloc(need);
+    fill_i32(h->stream, gbits.as<int32_t>(), (int64_t)(need / 4), 0);
+    BCLX_HIP(hipStreamSynchronize(h->stream));
+  }
 }
 
 template struct FlatRows<float, int32_t>;
@@ -509,6 +528,36 @@ bclx_status bclx_spgemm_symbolic(bclx_handle* h, const bclx_csr* A, const bclx_c
     else p->impl = std::make_unique<TwoPhasePlan<double, int64_t>>(h, A, B);
     *nnz_c = p->impl->symbolic();
     *plan = p.release();
+  });
+}
+
+bclx_status bclx_spgemm(bclx_handle* h, const bclx_csr* A, const bclx_csr* B, bclx_csr* C) {
+  return guarded([&] {
+    BCLX_REQUIRE(h && C, BCLX_ERR_INVALID, "null handle or C");
+    BCLX_REQUIRE(C->ptr_width == 4 || C->ptr_width == 8, BCLX_ERR_INVALID, "C->ptr_width must be 4 or 8");
+    check_pair(A, B);
+    set_device(h);
+    std::unique_ptr<PlanBase> plan;
+    const bool f64 = A->dtype == BCLX_F64;
+    const bool w8 = A->ptr_width == 8;
+    if (!f64 && !w8) plan = std::make_unique<TwoPhasePlan<float, int32_t>>(h, A, B);
+    else if (!f64 && w8) plan = std::make_unique<TwoPhasePlan<float, int64_t>>(h, A, B);
+    else if (f64 && !w8) plan = std::make_unique<TwoPhasePlan<double, int32_t>>(h, A, B);
+    else plan = std::make_unique<TwoPhasePlan<double, int64_t>>(h, A, B);
+    const int64_t nnz = plan->symbolic();
+    BCLX_REQUIRE(C->ptr_width == 8 || nnz <= INT32_MAX, BCLX_ERR_OVERFLOW,
+                 "nnz(C) = " + std::to_string(nnz) + " does not fit an int32 row_ptr; request ptr_width 8");
+    // the outputs belong to the caller: they are released here only if this call fails
+    const size_t vsz = f64 ? 8 : 4;
+    DevBuf rp, col, val;
+    rp.alloc((size_t)C->ptr_width * (size_t)(A->m + 1));
+    col.alloc(4 * (size_t)std::max<int64_t>(nnz, 1));
+    val.alloc(vsz * (size_t)std::max<int64_t>(nnz, 1));
+    bclx_csr out{A->m, B->n, nnz, C->ptr_width, A->dtype, rp.p, col.as<int32_t>(), val.p};
+    plan->numeric(&out);
+    BCLX_HIP(hipStreamSynchronize(h->stream));
+    *C = out;
+    rp.p = col.p = val.p = nullptr;  // handed over
   });
 }
 

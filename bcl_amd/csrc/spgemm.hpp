@@ -32,8 +32,7 @@ struct FlatRows {
   RowsDev<T, RP> R{};
   long long* scan_partial = nullptr;
   int64_t* scan_total = nullptr;
-  DevBuf gbits;
-  bool gbits_zeroed = false;
+  DevBuf gbits;  // per-workgroup column bitmaps of the wide-tile heavy kernel (kept zeroed between rows)
 
   void setup(bclx_handle* h, const std::vector<TileDev<T, RP>>& tiles, bool with_cap);
   void upload_tiles();
@@ -50,6 +49,7 @@ struct FlatRows {
                int64_t max_out, int32_t* chunk_row);
   // heavy rows: allocate the zeroed accumulators of the heavy kernel (before any graph capture)
   void prepare_heavy();
+  bool heavy_lds_bitmap() const;  // wide-tile heavy kernel: bitmap in LDS (else per-workgroup global bitmaps)
   int64_t read_stat(int q);
   void read_counts(hipStream_t s, int32_t* dst);  // kNumBins bin counts (synchronises s)
 };

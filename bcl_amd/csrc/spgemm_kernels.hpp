@@ -628,6 +628,8 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
     } else {
       const int cnt = s_cnt[g];
       const bool ovf = MODE == 2 && s_flag[g];
+      // A row refused for lack of output room keeps nnz = 0 (and a harmless offset), so later passes that copy
+      // nnz entries from its offset (compaction) stay inside the arrays; the refusal is flagged in stats[3].
       if (active && lane == 0) {
         if (MODE == 2) {
           if (ovf) {
@@ -635,12 +637,16 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
             R.ovf_items[slot] = fr;
             atomicAdd(&R.stats[2], 1ull);
           } else {
-            R.nnz[fr] = cnt;
+            const bool fits = R.off[fr] + cnt <= R.out_limit;
+            R.nnz[fr] = fits ? cnt : 0;
+            if (!fits) R.stats[3] = 1ull;
           }
         } else if (MODE == 3) {
           s_base[g] = (long long)atomicAdd(R.cursor, (unsigned long long)cnt);
-          R.off[fr] = s_base[g];
-          R.nnz[fr] = cnt;
+          const bool fits = s_base[g] + cnt <= R.out_limit;
+          R.off[fr] = fits ? s_base[g] : 0;
+          R.nnz[fr] = fits ? cnt : 0;
+          if (!fits) R.stats[3] = 1ull;
         }
       }
       bool ranked = false;
@@ -954,11 +960,12 @@ __global__ __launch_bounds__(64 * kEscWaves) void k_esc(RowsDev<T, RP> R, int bi
         o = (long long)atomicAdd(R.cursor, (unsigned long long)cnt);
         R.off[fr] = o;
       }
-      if (o + cnt > R.out_limit) {
+      if (o + cnt > R.out_limit) {  // refused: nothing is written and, after this pass, the row holds 0 entries
         R.stats[3] = 1ull;
         o = -1;
+        if (MODE != 1) R.off[fr] = 0;
       }
-      R.nnz[fr] = cnt;
+      R.nnz[fr] = (o < 0 && MODE != 1) ? 0 : cnt;
     }
     o = __shfl(o, 0, 64);
 #pragma unroll
@@ -1122,11 +1129,12 @@ __global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int b
         o = (long long)atomicAdd(R.cursor, (unsigned long long)cnt);
         R.off[fr] = o;
       }
-      if (o + cnt > R.out_limit) {
+      if (o + cnt > R.out_limit) {  // refused: nothing is written and, after this pass, the row holds 0 entries
         R.stats[3] = 1ull;
         o = -1;
+        if (MODE != 1) R.off[fr] = 0;
       }
-      R.nnz[fr] = cnt;
+      R.nnz[fr] = (o < 0 && MODE != 1) ? 0 : cnt;
       s_o = o;
     }
     __syncthreads();
@@ -1425,13 +1433,13 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
           b = (long long)atomicAdd(R.cursor, (unsigned long long)total);
           R.off[fr] = b;
         }
-        if (b + total > R.out_limit) {
+        if (b + total > R.out_limit) {  // refused: nothing is written and, after this pass, the row holds 0 entries
           R.stats[3] = 1ull;
           b = -1;
-          if (MODE != 1) R.off[fr] = -1;
+          if (MODE != 1) R.off[fr] = 0;
         }
         s_base = b;
-        R.nnz[fr] = (int)total;
+        R.nnz[fr] = (b < 0 && MODE != 1) ? 0 : (int)total;
         atomicAdd(&R.stats[1], 1ull);
       }
       __syncthreads();
@@ -1592,12 +1600,13 @@ __global__ __launch_bounds__(kHeavyThreads) void k_heavy(RowsDev<T, RP> R, int b
           b = (long long)atomicAdd(R.cursor, (unsigned long long)total);
           R.off[fr] = b;
         }
-        if (b + total > R.out_limit) {
+        if (b + total > R.out_limit) {  // refused: nothing is written and, after this pass, the row holds 0 entries
           R.stats[3] = 1ull;
           b = -1;
+          if (MODE != 1) R.off[fr] = 0;
         }
         s_base = b;
-        R.nnz[fr] = (int)total;
+        R.nnz[fr] = (b < 0 && MODE != 1) ? 0 : (int)total;
       }
       __syncthreads();
       const bool fits = s_base >= 0;

@@ -89,6 +89,19 @@ __global__ __launch_bounds__(256) void k_ovf_bin(RowsDev<T, RP> R) {
   }
 }
 
+// Flags stats[3] when the predicted slots of this execution need more room than the slot region the plan sized
+// (inputs whose structure grew since the plan): rows past the region would overlap the bump-allocated ones.
+__global__ void k_check_slots(const int64_t* total, long long region, unsigned long long* flag) {
+  if (threadIdx.x == 0 && *total > region) *flag = 1ull;
+}
+
+// Flags stats[3] when rows were binned into a bin that this pipeline does not launch (empty when the plan was sized,
+// so left out of the graph): inputs whose structure changed since the plan would otherwise lose those rows.
+__global__ void k_check_bins(const int32_t* bin_count, unsigned launched, unsigned long long* flag) {
+  const int b = threadIdx.x;
+  if (b > 0 && b < kNumBins && !((launched >> b) & 1u) && bin_count[b * kCountStride] > 0) *flag = 1ull;
+}
+
 // Per-product row_ptr (int64, starting at 0) from flat offsets: product t's row_ptr lives at rp[flat0 + t ...].
 template <typename T, typename RP>
 __global__ __launch_bounds__(kRowBlock) void k_tile_rowptr(RowsDev<T, RP> R, const int64_t* off, int64_t* rp) {
@@ -209,6 +222,11 @@ struct StagePlan : StageBase {
   }
   // Run the LDS bins and the heavy bin of `mode_k` on the side streams (largest expected first).
   void run_bins(hipStream_t s, int mode_k, const int32_t* counts, bool persistent) {
+    unsigned launched = 0;
+    for (int b = 1; b < kNumBins; b++)
+      if (counts[b] > 0) launched |= 1u << b;
+    hipLaunchKernelGGL(k_check_bins, dim3(1), dim3(64), 0, s, (const int32_t*)F.R.bin_count, launched, F.R.stats + 3);
+    BCLX_HIP(hipGetLastError());
     fork(s);
     int q = 0;
     if (counts[kHeavyBin] > 0) F.run_bin(side[q++ % side.size()], mode_k, kHeavyBin, persistent ? -1 : counts[kHeavyBin]);
@@ -247,6 +265,9 @@ struct StagePlan : StageBase {
       fill_u64(s, F.R.cursor, 1, (unsigned long long)main_cap);
       predict(s, false);  // products were counted by the sizing run
       F.scan(s, F.R.cap, F.R.off, 0);
+      hipLaunchKernelGGL(k_check_slots, dim3(1), dim3(64), 0, s, (const int64_t*)F.scan_total, (long long)main_cap,
+                         F.R.stats + 3);
+      BCLX_HIP(hipGetLastError());
       run_bins(s, 2, counts1, persistent);
       fill_i32(s, F.R.bin_count, kNumBins * kCountStride, 0);
       hipLaunchKernelGGL((k_ovf_bin<T, RP>), dim3(std::max<unsigned>(1, std::min<unsigned>(div_up(counts_ovf, 256), 1024))),
@@ -387,7 +408,7 @@ struct StagePlan : StageBase {
     final_off.alloc(sizeof(int64_t) * (F.nflat + 1));
     tile_rp.alloc(sizeof(int64_t) * (F.nflat + npairs + 1));
     // heavy-row accumulators of tiles wider than the LDS bitmap are allocated before capture
-    if ((F.max_n + 31) / 32 > kHRWords) ensure_dense_slots(h, h->num_cus, (size_t)F.max_n);
+    F.prepare_heavy();
     sizing_run();
     if (opts.use_graph) {
       // capture on a plan-owned stream (the handle's stream may be the legacy default stream, which cannot be
@@ -430,6 +451,13 @@ struct StagePlan : StageBase {
       C->vals = c_val[p].p;
       return;
     }
+    // A row refused for lack of room (inputs whose structure outgrew what the plan was sized for) was not written:
+    // no product of this execution is handed out.
+    unsigned long long refused = 0;
+    BCLX_HIP(hipMemcpy(&refused, F.R.stats + 3, sizeof(refused), hipMemcpyDeviceToHost));
+    BCLX_REQUIRE(refused == 0, BCLX_ERR_OVERFLOW,
+                 "stage output exceeded the capacity sized at plan time (the inputs' sparsity structure grew since "
+                 "the plan was made: make a new plan)");
     const TileDev<T, RP>& t = F.tiles[p];
     const int64_t* off = mode == 1 ? final_off.as<int64_t>() : F.R.off;
     int64_t lo = 0, hi = 0;
@@ -445,7 +473,7 @@ struct StagePlan : StageBase {
 
   void stats(int64_t* out, int n) override {
     BCLX_HIP(hipStreamSynchronize(h->stream));
-    int64_t st[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t rows = 0;
     if (mode == 0) {
       for (int p = 0; p < npairs; p++) {
@@ -472,9 +500,10 @@ struct StagePlan : StageBase {
       BCLX_REQUIRE(v[3] == 0, BCLX_ERR_OVERFLOW,
                    "stage output exceeded the capacity sized at plan time (inputs changed since the plan?)");
       for (auto& t : F.tiles) rows += t.m;
+      for (int b = kEscBin0; b < kEscBin0 + kEscBins; b++) st[7] += counts1[b];
     }
     st[5] = rows;
-    for (int q = 0; q < n && q < 7; q++) out[q] = st[q];
+    for (int q = 0; q < n && q < 8; q++) out[q] = st[q];
   }
 };
 

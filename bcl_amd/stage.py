@@ -23,7 +23,7 @@ def full_tuning(t) -> np.ndarray:
     """A complete BCLX_TUNING_LEN table: the given leading knobs, defaults for the rest."""
     t = [int(x) for x in t]
     return np.array(t + list(TUNING_DEFAULTS[len(t):]), np.int32)
-STAT_NAMES = ["products", "nnz", "overflow_rows", "predicted_rows", "capacity", "rows", "heavy_rows"]
+STAT_NAMES = ["products", "nnz", "overflow_rows", "predicted_rows", "capacity", "rows", "heavy_rows", "sorted_rows"]
 
 
 class StagePlan:
@@ -55,8 +55,8 @@ class StagePlan:
         call("bclx_stage_execute", self.p)
 
     def stats(self) -> dict:
-        out = np.zeros(7, np.int64)
-        call("bclx_stage_stats", self.p, out.ctypes.data_as(ctypes.c_void_p), 7)
+        out = np.zeros(len(STAT_NAMES), np.int64)
+        call("bclx_stage_stats", self.p, out.ctypes.data_as(ctypes.c_void_p), len(STAT_NAMES))
         return dict(zip(STAT_NAMES, [int(x) for x in out]))
 
     def result_desc(self, p: int) -> bclx_csr:

@@ -73,6 +73,16 @@ BCLX_API bclx_status bclx_destroy(bclx_handle* h);
 BCLX_API bclx_status bclx_set_stream(bclx_handle* h, void* stream);
 BCLX_API bclx_status bclx_synchronize(bclx_handle* h);
 
+/* Device allocator of a handle, for the library's own buffers (plans, scratch) and for the outputs bclx_spgemm
+ * allocates. alloc(bytes, user) returns device memory or NULL (the call that needed it then fails with
+ * BCLX_ERR_ALLOC); free(ptr, user) releases it. Both NULL: hipMalloc / hipFree (the default). Replaces BCL's
+ * Allocator template parameter (bcl/containers/sequential/CSRMatrix.hpp:32-36, rebound for the index arrays at
+ * :41-42; bcl_allocator, bcl/core/alloc.hpp:60-105). Buffers are freed through the allocator that made them, so its
+ * functions must stay valid until every plan made under them is destroyed. */
+typedef void* (*bclx_alloc_fn)(size_t bytes, void* user);
+typedef void (*bclx_free_fn)(void* ptr, void* user);
+BCLX_API bclx_status bclx_set_allocator(bclx_handle* h, bclx_alloc_fn alloc, bclx_free_fn free_fn, void* user);
+
 /* ---- SpGEMM C = A * B, two phases (exact-zero sums are kept, as BCL does) ---------------------------------
  * symbolic: row analysis (products per row), binning and an exact per-row count of C's pattern; returns
  * nnz(C). numeric: fills caller-allocated C (row_ptr m+1 of C->ptr_width, col_idx/vals of nnz) with sorted
@@ -82,6 +92,11 @@ BCLX_API bclx_status bclx_spgemm_symbolic(bclx_handle* h, const bclx_csr* A, con
                                           bclx_spgemm_plan** plan, int64_t* nnz_c);
 BCLX_API bclx_status bclx_spgemm_numeric(bclx_spgemm_plan* plan, bclx_csr* C);
 BCLX_API bclx_status bclx_spgemm_plan_destroy(bclx_spgemm_plan* plan);
+/* C = A * B in one call with library-allocated outputs (the way CSRMatrix::dot returns a fresh matrix,
+ * bcl/containers/sequential/CSRMatrix.hpp:229-235): C->row_ptr (m+1 entries of C->ptr_width), C->col_idx and C->vals
+ * (nnz entries each, at least one element) are allocated with the handle's allocator and owned by the caller, who
+ * releases them with the matching free function (hipFree by default). Sets C->m, C->n, C->nnz and C->dtype. */
+BCLX_API bclx_status bclx_spgemm(bclx_handle* h, const bclx_csr* A, const bclx_csr* B, bclx_csr* C);
 /* Per-row products (int64, m entries) and, after symbolic, nnz(C_i) (int32) of a plan, copied to host. */
 BCLX_API bclx_status bclx_spgemm_plan_rows(bclx_spgemm_plan* plan, int64_t* h_products, int32_t* h_row_nnz);
 
@@ -114,10 +129,15 @@ typedef struct {
 BCLX_API bclx_status bclx_stage_plan_create(bclx_handle* h, int npairs, const bclx_csr* A, const bclx_csr* B,
                                             const bclx_stage_opts* opts, bclx_stage_plan** plan);
 BCLX_API bclx_status bclx_stage_execute(bclx_stage_plan* plan);
-/* Device view of product p's result (row_ptr int64, pointers into the plan's arena; valid until destroy). */
+/* Device view of product p's result (row_ptr int64, pointers into the plan's arena; valid until destroy).
+ * A plan is sized for the sparsity structure of its inputs; values may change between executions. If the structure
+ * changes, an execution that still fits the plan's bins and capacities recomputes the new products; one that does
+ * not refuses the rows it cannot place, and this call then fails with BCLX_ERR_OVERFLOW (make a new plan). Mode 0
+ * fails in bclx_stage_execute instead. */
 BCLX_API bclx_status bclx_stage_result(bclx_stage_plan* plan, int p, bclx_csr* C);
 /* Stats: [0] total products, [1] total nnz(C), [2] overflow rows, [3] predicted rows, [4] arena capacity,
- * [5] rows, [6] heavy rows. */
+ * [5] rows, [6] heavy rows, [7] sorted (ESC) rows. Fails with BCLX_ERR_OVERFLOW, as bclx_stage_result does, when the
+ * last execution refused rows for lack of room. */
 BCLX_API bclx_status bclx_stage_stats(bclx_stage_plan* plan, int64_t* h_stats, int nstats);
 BCLX_API bclx_status bclx_stage_plan_destroy(bclx_stage_plan* plan);
 
