@@ -66,6 +66,10 @@ SIGNATURES = [
     ("bclx_spmm", _I32, [_P, ctypes.POINTER(bclx_csr), _P, _I32, _I64, _P, _I64, _I64, _F32, _F32]),
     ("bclx_gemm", _I32, [_P, _I32, _I32, _I64, _I64, _I64, _F32, _P, _I64, _P, _I64, _F32, _P, _I64, _I32]),
     ("bclx_gemm_f64_ref", _I32, [_P, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64]),
+    ("bclx_mm_info", _I32, [ctypes.c_char_p, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    ("bclx_mm_read", _I32, [ctypes.c_char_p, _I32, _I32, _I32, ctypes.POINTER(bclx_csr)]),
+    ("bclx_mm_write", _I32, [ctypes.c_char_p, ctypes.POINTER(bclx_csr)]),
+    ("bclx_host_free", None, [_P]),
 ]
 
 _lib = None

@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-fvisibility=hidden", "-I" + os.path.join(ROOT, "include"), "-Wno-unused-result",
 ]
 HIP_SOURCES = ["core.hip", "spgemm.hip", "stage.hip", "mlp.hip", "spmm.hip", "gemm.hip", "accumulate.hip"]
-CPP_SOURCES: list = []
+CPP_SOURCES = ["mmio.cpp"]  # host-only parts of the C ABI (g++)
 
 
 def _run(cmd):
@@ -49,13 +49,14 @@ def build_bclx(jobs: int = 8) -> str:
     for src in CPP_SOURCES:
         obj = os.path.join(OBJDIR, src + ".o")
         objs.append(obj)
-        cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-I" + os.path.join(ROOT, "include"),
+        cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-fvisibility=hidden", "-Wall",
+                     "-I" + os.path.join(ROOT, "include"),
                      "-c", os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     out = os.path.join(LIBDIR, "libbclx.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -87,6 +87,87 @@ def read_binary(path: str, val_dtype=np.float32, index_dtype=np.int32) -> CSR:
     return CSR(m, n, rp, col, vals)
 
 
+def read_binary_mmap(path: str, val_dtype=np.float32, index_dtype=np.int32) -> CSR:
+    """BCL's binary CSR file mapped into memory without reading it (CSRMatrixMemoryMapped,
+    bcl/containers/sequential/CSRMatrixMemoryMapped.hpp:55-90): the arrays are read-only views of the file, paged
+    in as they are used (a slice or a device upload touches only what it needs). Column indices and row offsets keep
+    the file's index type."""
+    head = np.memmap(path, np.uint64, mode="r", shape=(3,))
+    m, n, nnz = (int(x) for x in head)
+    del head
+    vsz, isz = np.dtype(val_dtype).itemsize, np.dtype(index_dtype).itemsize
+    vals = np.memmap(path, val_dtype, mode="r", offset=24, shape=(nnz,)) if nnz else np.zeros(0, val_dtype)
+    col = (np.memmap(path, index_dtype, mode="r", offset=24 + nnz * vsz, shape=(nnz,)) if nnz
+           else np.zeros(0, index_dtype))
+    rp = np.memmap(path, index_dtype, mode="r", offset=24 + nnz * (vsz + isz), shape=(m + 1,))
+    return CSR(m, n, rp, col, vals)
+
+
+def get_slice(a: CSR, imin: int, imax: int, jmin: int, jmax: int) -> CSR:
+    """The submatrix [imin, imax) x [jmin, jmax) with columns shifted by jmin (CSRMatrix::get_slice_impl_,
+    bcl/containers/sequential/CSRMatrix.hpp:238-276; CSRMatrixMemoryMapped::get_slice_impl_,
+    CSRMatrixMemoryMapped.hpp:92-145): bounds are clipped to the matrix, entries keep their order within a row.
+    Vectorised over the rows of the slice, so a slice of a memory-mapped file reads only those rows."""
+    imax, jmax = min(imax, a.m), min(jmax, a.n)
+    imin, jmin = max(imin, 0), max(jmin, 0)
+    assert imin <= imax and jmin <= jmax
+    lo, hi = int(a.row_ptr[imin]), int(a.row_ptr[imax])
+    col = np.asarray(a.col_idx[lo:hi]).astype(np.int64)
+    keep = (col >= jmin) & (col < jmax)
+    rows = np.repeat(np.arange(imax - imin), np.diff(np.asarray(a.row_ptr[imin:imax + 1]).astype(np.int64)))
+    rp = np.zeros(imax - imin + 1, np.int64)
+    np.cumsum(np.bincount(rows[keep], minlength=imax - imin), out=rp[1:])
+    return CSR(imax - imin, jmax - jmin, rp, (col[keep] - jmin).astype(np.int32), np.asarray(a.vals[lo:hi])[keep])
+
+
+def matrix_market_info(path: str):
+    """(m, n, stored entries) of a MatrixMarket file's size line (matrix_io::matrix_market_info,
+    bcl/containers/sequential/matrix_io.hpp:68-98)."""
+    import ctypes
+    from ._lib import call
+    v = [ctypes.c_int64() for _ in range(3)]
+    call("bclx_mm_info", path.encode(), 1, *(ctypes.byref(x) for x in v))
+    return tuple(int(x.value) for x in v)
+
+
+def read_matrix_market(path: str, one_indexed: bool = True, dtype=np.float32) -> CSR:
+    """A MatrixMarket coordinate file as CSR, sorted by (i, j), the first occurrence of a repeated entry kept and
+    symmetric files mirrored (CSRMatrix::read_MatrixMarket over matrix_io::mmread,
+    bcl/containers/sequential/CSRMatrix.hpp:369-411, matrix_io.hpp:221-357), read by libbclx in linear time.
+    Raises BclxError with the reference's messages on malformed files."""
+    import ctypes
+    from ._lib import BCLX_F32, BCLX_F64, bclx_csr, call, lib
+    f64 = np.dtype(dtype) == np.float64
+    out = bclx_csr()
+    call("bclx_mm_read", path.encode(), int(one_indexed), BCLX_F64 if f64 else BCLX_F32, 8, ctypes.byref(out))
+    try:
+        m, nnz = out.m, out.nnz
+        rp = np.ctypeslib.as_array(ctypes.cast(out.row_ptr, ctypes.POINTER(ctypes.c_int64)), shape=(m + 1,)).copy()
+        if nnz:
+            col = np.ctypeslib.as_array(ctypes.cast(out.col_idx, ctypes.POINTER(ctypes.c_int32)), shape=(nnz,)).copy()
+            ct = ctypes.c_double if f64 else ctypes.c_float
+            val = np.ctypeslib.as_array(ctypes.cast(out.vals, ctypes.POINTER(ct)), shape=(nnz,)).copy()
+        else:
+            col, val = np.zeros(0, np.int32), np.zeros(0, np.float64 if f64 else np.float32)
+    finally:
+        for ptr in (out.row_ptr, out.col_idx, out.vals):
+            lib().bclx_host_free(ptr)
+    return CSR(out.m, out.n, rp, col, val)
+
+
+def write_matrix_market(a: CSR, path: str) -> None:
+    """CSRMatrix::write_MatrixMarket's format (bcl/containers/sequential/CSRMatrix.hpp:532-548), written by
+    libbclx."""
+    import ctypes
+    from ._lib import BCLX_F32, BCLX_F64, bclx_csr, call
+    rp = np.ascontiguousarray(a.row_ptr, np.int64)
+    col = np.ascontiguousarray(a.col_idx, np.int32)
+    f64 = a.vals.dtype == np.float64
+    val = np.ascontiguousarray(a.vals, np.float64 if f64 else np.float32)
+    d = bclx_csr(a.m, a.n, a.nnz, 8, BCLX_F64 if f64 else BCLX_F32, rp.ctypes.data, col.ctypes.data, val.ctypes.data)
+    call("bclx_mm_write", path.encode(), ctypes.byref(d))
+
+
 def grid_tile_shape(m: int, n: int, grid_m: int, grid_n: int):
     """BlockCustom with Tile::div (bcl/containers/detail/Blocking.hpp:223-232): ceil(m/grid_m), ceil(n/grid_n)."""
     return (m + grid_m - 1) // grid_m, (n + grid_n - 1) // grid_n

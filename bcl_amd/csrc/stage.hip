@@ -133,7 +133,8 @@ struct StagePlan : StageBase {
   // mode 0
   std::vector<std::unique_ptr<TwoPhase<T, RP>>> per;  // one two-phase workspace per product
   std::vector<DevBuf> c_rp, c_col, c_val;
-  std::vector<int64_t> c_nnz;
+  std::vector<int64_t> c_nnz;  // capacity of each product's output (its nnz when the plan was made)
+  std::vector<int64_t> c_len;  // nnz of each product in the last execution
   // modes 1 / 2
   FlatRows<T, RP> F;
   DevBuf params, arena_col, arena_val, final_col, final_val, final_off, tile_rp, chunk_row;
@@ -189,8 +190,10 @@ struct StagePlan : StageBase {
   void plain_product(int p) {
     TwoPhase<T, RP>& W = *per[p];
     const int64_t tot = W.symbolic();
+    c_len[p] = 0;
     BCLX_REQUIRE(tot <= c_nnz[p], BCLX_ERR_OVERFLOW, "stage product grew since the plan was made");
     W.numeric(c_col[p].as<int>(), c_val[p].as<T>(), c_nnz[p], c_rp[p].p, 8);
+    c_len[p] = tot;
   }
 
   void setup_plain(const std::vector<TileDev<T, RP>>& tiles) {
@@ -198,10 +201,12 @@ struct StagePlan : StageBase {
     c_col.resize(npairs);
     c_val.resize(npairs);
     c_nnz.assign(npairs, 0);
+    c_len.assign(npairs, 0);
     for (int p = 0; p < npairs; p++) {
       per.push_back(std::make_unique<TwoPhase<T, RP>>(h, tiles[p], tn));
       const int64_t tot = per.back()->symbolic();  // sizes the outputs
       c_nnz[p] = tot;
+      c_len[p] = tot;
       c_col[p].alloc(sizeof(int) * std::max<int64_t>(tot, 1));
       c_val[p].alloc(sizeof(T) * std::max<int64_t>(tot, 1));
       c_rp[p].alloc(sizeof(int64_t) * (tiles[p].m + 1));
@@ -445,7 +450,7 @@ struct StagePlan : StageBase {
     if (mode == 0) {
       C->m = per[p]->F.tiles[0].m;
       C->n = per[p]->F.tiles[0].n;
-      C->nnz = c_nnz[p];
+      C->nnz = c_len[p];
       C->row_ptr = c_rp[p].p;
       C->col_idx = c_col[p].as<int>();
       C->vals = c_val[p].p;
@@ -481,10 +486,10 @@ struct StagePlan : StageBase {
         BCLX_HIP(hipMemcpy(v, per[p]->F.R.stats, sizeof(v), hipMemcpyDeviceToHost));
         st[0] += (int64_t)v[0];
         st[6] += (int64_t)v[1];
-        st[1] += c_nnz[p];
+        st[1] += c_len[p];
+        st[4] += c_nnz[p];
         rows += per[p]->F.tiles[0].m;
       }
-      st[4] = 0;
     } else {
       unsigned long long v[5];
       BCLX_HIP(hipMemcpy(v, F.R.stats, sizeof(v), hipMemcpyDeviceToHost));

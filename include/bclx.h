@@ -174,6 +174,23 @@ BCLX_API bclx_status bclx_gemm(bclx_handle* h, int32_t trans_a, int32_t trans_b,
 BCLX_API bclx_status bclx_gemm_f64_ref(bclx_handle* h, int64_t m, int64_t n, int64_t k, const float* d_A,
                                        int64_t lda, const float* d_B, int64_t ldb, double* d_C, int64_t ldc);
 
+/* ---- MatrixMarket files (host memory, no GPU needed) -------------------------------------------------------
+ * bclx_mm_info: m, n and the stored entry count from the size line
+ *     (BCL::matrix_io::matrix_market_info, bcl/containers/sequential/matrix_io.hpp:68-98).
+ * bclx_mm_read: a coordinate file (pattern / numeric field, general / symmetric) as a host CSR matrix sorted by
+ *     (i, j), the first occurrence of a repeated (i, j) kept (BCL::matrix_io::mmread, matrix_io.hpp:221-357, and
+ *     CSRMatrix::read_MatrixMarket, bcl/containers/sequential/CSRMatrix.hpp:369-411), in time linear in the file.
+ *     out->row_ptr (m + 1 entries of ptr_width), out->col_idx and out->vals (dtype) are malloc'd host arrays owned by
+ *     the caller (bclx_host_free). Errors (BCLX_ERR_INVALID) carry the reference's messages: not a Matrix Market
+ *     file, unsupported matrix type, nonzero out of bounds, more nonzeros than reported.
+ * bclx_mm_write: a host CSR matrix in the reference's output format (CSRMatrix::write_MatrixMarket,
+ *     CSRMatrix.hpp:532-548): "integer general" banner, 1-based indices, values printed with "%f". */
+BCLX_API bclx_status bclx_mm_info(const char* path, int32_t one_indexed, int64_t* m, int64_t* n, int64_t* nnz);
+BCLX_API bclx_status bclx_mm_read(const char* path, int32_t one_indexed, int32_t dtype, int32_t ptr_width,
+                                  bclx_csr* out);
+BCLX_API bclx_status bclx_mm_write(const char* path, const bclx_csr* host_csr);
+BCLX_API void bclx_host_free(void* p);
+
 /* Library version string. */
 BCLX_API const char* bclx_version(void);
 

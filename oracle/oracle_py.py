@@ -85,6 +85,23 @@ def ref():
         R.ref_binary_roundtrip.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
         R.ref_binary_roundtrip.restype = _I64
         R.ref_spmm_f32.restype = None
+        for nm in ("ref_mm_read_f32", "ref_mm_read_f64"):
+            getattr(R, nm).argtypes = [ctypes.c_char_p, ctypes.c_int]
+            getattr(R, nm).restype = _P
+        R.ref_error.argtypes = []
+        R.ref_error.restype = ctypes.c_char_p
+        R.ref_result_m.argtypes = [_P, ctypes.c_int]
+        R.ref_result_m.restype = _I64
+        R.ref_result_n.argtypes = [_P, ctypes.c_int]
+        R.ref_result_n.restype = _I64
+        R.ref_mm_info.argtypes = [ctypes.c_char_p, _P]
+        R.ref_mm_info.restype = None
+        R.ref_mm_write_f32.argtypes = [ctypes.c_char_p, _I64, _I64, _I64, _P, _P, _P]
+        R.ref_mm_write_f32.restype = None
+        R.ref_mmap_slice_f32.argtypes = [ctypes.c_char_p, _I64, _I64, _I64, _I64]
+        R.ref_mmap_slice_f32.restype = _P
+        R.ref_slice_f32.argtypes = [_I64, _I64, _I64, _P, _P, _P, _I64, _I64, _I64, _I64]
+        R.ref_slice_f32.restype = _P
         _ref = R
     return _ref
 
@@ -277,3 +294,41 @@ def ref_spmm(a: CSR, b: np.ndarray, c: np.ndarray) -> None:
     av = np.ascontiguousarray(a.vals, np.float32)
     ref().ref_spmm_f32(a.m, b.shape[1], a.n, a.nnz, _ptr(c), _ptr(b), _ptr(av), _ptr(arp), _ptr(acol), b.shape[1],
                        c.shape[1])
+
+
+class RefError(RuntimeError):
+    """An exception the reference threw (its message)."""
+
+
+def ref_mm_read(path: str, one_indexed: bool = True, dtype=np.float32) -> CSR:
+    """The reference CSRMatrix<T, int>::read_MatrixMarket (over matrix_io::mmread); RefError on its exceptions."""
+    f64 = dtype == np.float64
+    R = ref()
+    r = (R.ref_mm_read_f64 if f64 else R.ref_mm_read_f32)(path.encode(), int(one_indexed))
+    if not r:
+        raise RefError(R.ref_error().decode())
+    return _ref_take(r, R.ref_result_m(r, int(f64)), R.ref_result_n(r, int(f64)), f64)
+
+
+def ref_mm_info(path: str):
+    out = np.zeros(3, np.int64)
+    ref().ref_mm_info(path.encode(), _ptr(out))
+    return tuple(int(x) for x in out)
+
+
+def ref_mm_write(path: str, a: CSR) -> None:
+    ref().ref_mm_write_f32(path.encode(), a.m, a.n, a.nnz, _ptr(_i32(a.row_ptr)), _ptr(_i32(a.col_idx)),
+                           _ptr(np.ascontiguousarray(a.vals, np.float32)))
+
+
+def ref_mmap_slice(path: str, imin: int, imax: int, jmin: int, jmax: int) -> CSR:
+    """The reference CSRMatrixMemoryMapped<float, int>(path).get_slice_impl_(imin, imax, jmin, jmax)."""
+    r = ref().ref_mmap_slice_f32(path.encode(), imin, imax, jmin, jmax)
+    return _ref_take(r, ref().ref_result_m(r, 0), ref().ref_result_n(r, 0), False)
+
+
+def ref_slice(a: CSR, imin: int, imax: int, jmin: int, jmax: int) -> CSR:
+    """The reference CSRMatrix<float, int>::get_slice_impl_ of an in-memory matrix."""
+    r = ref().ref_slice_f32(a.m, a.n, a.nnz, _ptr(_i32(a.row_ptr)), _ptr(_i32(a.col_idx)),
+                            _ptr(np.ascontiguousarray(a.vals, np.float32)), imin, imax, jmin, jmax)
+    return _ref_take(r, ref().ref_result_m(r, 0), ref().ref_result_n(r, 0), False)

@@ -5,6 +5,8 @@
 //   BCL::gemm(CSRMatrix, CSRMatrix, CSRMatrix)       bcl/containers/algorithms/spgemm.hpp:40-76
 //   BCL::spmm_wrapper(...)                            bcl/containers/algorithms/spgemm.hpp:140-149
 //   BCL::SparseHashAccumulator::get_matrix(m, n)     bcl/containers/sequential/SparseHashAccumulator.hpp:101-182
+//   CSRMatrix::read_MatrixMarket / write_MatrixMarket, matrix_io::matrix_market_info,
+//   CSRMatrix::get_slice_impl_ and CSRMatrixMemoryMapped::get_slice_impl_ (the tile ingest and file formats)
 // so the C oracle can be checked bit for bit against the reference and the CPU baseline can time it.
 // The headers pull in the MPI backend declarations (bcl/bcl.hpp:19-24); none of it runs on these paths, and
 // the build drops the unused MPI-calling functions (-ffunction-sections, --gc-sections), so no MPI library
@@ -13,10 +15,12 @@
 #include <bcl/containers/sequential/CSRMatrix.hpp>
 #include <bcl/containers/algorithms/spgemm.hpp>
 #include <bcl/containers/sequential/SparseHashAccumulator.hpp>
+#include <bcl/containers/sequential/CSRMatrixMemoryMapped.hpp>
 
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace {
@@ -56,6 +60,22 @@ void* accumulate(int64_t m, int64_t n, int ntiles, const int64_t* nnzs, const in
   auto c = acc.get_matrix(m, n);
   auto* r = new RefResult<T>{(int64_t)c.m_, (int64_t)c.n_, (int64_t)c.nnz_, c.row_ptr_, c.col_ind_, c.vals_};
   return r;
+}
+
+thread_local std::string g_ref_error;
+
+// CSRMatrix::read_MatrixMarket (CSRMatrix.hpp:369-411 over matrix_io::mmread, matrix_io.hpp:221-357). NULL on an
+// exception, whose message ref_error() returns.
+template <typename T>
+void* mm_read(const char* path, int one_indexed) {
+  try {
+    BCL::CSRMatrix<T, int> a(0, 0);
+    a.read_MatrixMarket(path, one_indexed != 0);
+    return new RefResult<T>{(int64_t)a.m_, (int64_t)a.n_, (int64_t)a.nnz_, a.row_ptr_, a.col_ind_, a.vals_};
+  } catch (const std::exception& e) {
+    g_ref_error = e.what();
+    return nullptr;
+  }
 }
 
 }  // namespace
@@ -108,6 +128,47 @@ REF_API int64_t ref_binary_roundtrip(const char* in, const char* out) {
   a.read_Binary(in);
   a.write_Binary(out);
   return (int64_t)a.nnz_;
+}
+
+REF_API void* ref_mm_read_f32(const char* path, int one_indexed) { return mm_read<float>(path, one_indexed); }
+REF_API void* ref_mm_read_f64(const char* path, int one_indexed) { return mm_read<double>(path, one_indexed); }
+REF_API const char* ref_error() { return g_ref_error.c_str(); }
+REF_API int64_t ref_result_m(void* r, int is_f64) {
+  return is_f64 ? static_cast<RefResult<double>*>(r)->m : static_cast<RefResult<float>*>(r)->m;
+}
+REF_API int64_t ref_result_n(void* r, int is_f64) {
+  return is_f64 ? static_cast<RefResult<double>*>(r)->n : static_cast<RefResult<float>*>(r)->n;
+}
+
+// matrix_io::matrix_market_info (matrix_io.hpp:68-98): m, n, nnz of the size line.
+REF_API void ref_mm_info(const char* path, int64_t* out) {
+  auto s = BCL::matrix_io::matrix_market_info(path);
+  out[0] = (int64_t)s.shape[0];
+  out[1] = (int64_t)s.shape[1];
+  out[2] = (int64_t)s.nnz;
+}
+
+// CSRMatrix::write_MatrixMarket (CSRMatrix.hpp:532-548) of a float/int matrix.
+REF_API void ref_mm_write_f32(const char* path, int64_t m, int64_t n, int64_t nnz, const int* rp, const int* col,
+                              const float* val) {
+  auto a = make_csr<float>(m, n, nnz, rp, col, val);
+  a.write_MatrixMarket(path);
+}
+
+// CSRMatrixMemoryMapped<float, int>(fname).get_slice_impl_(imin, imax, jmin, jmax)
+// (bcl/containers/sequential/CSRMatrixMemoryMapped.hpp:55-145) of a binary CSR file.
+REF_API void* ref_mmap_slice_f32(const char* path, int64_t imin, int64_t imax, int64_t jmin, int64_t jmax) {
+  BCL::CSRMatrixMemoryMapped<float, int> f(path);
+  auto c = f.get_slice_impl_(imin, imax, jmin, jmax);
+  return new RefResult<float>{(int64_t)c.m_, (int64_t)c.n_, (int64_t)c.nnz_, c.row_ptr_, c.col_ind_, c.vals_};
+}
+
+// CSRMatrix::get_slice_impl_ (CSRMatrix.hpp:238-276) of an in-memory float/int matrix.
+REF_API void* ref_slice_f32(int64_t m, int64_t n, int64_t nnz, const int* rp, const int* col, const float* val,
+                            int64_t imin, int64_t imax, int64_t jmin, int64_t jmax) {
+  auto a = make_csr<float>(m, n, nnz, rp, col, val);
+  auto c = a.get_slice_impl_(imin, imax, jmin, jmax);
+  return new RefResult<float>{(int64_t)c.m_, (int64_t)c.n_, (int64_t)c.nnz_, c.row_ptr_, c.col_ind_, c.vals_};
 }
 
 REF_API void ref_spmm_f32(int64_t m, int64_t n, int64_t k, int64_t nnz, float* c, float* b, float* a_val, int* a_rp,

@@ -1,7 +1,7 @@
 """Write the golden fixtures of the tile multiply from the reference's own CPU code.
 
-Runs in the container that holds the reference sources: the reference's BCL::gemm(CSRMatrix...), spmm_wrapper and
-SparseHashAccumulator::get_matrix are compiled from those sources into oracle/_ref (make -C oracle ref) and called
+Runs in the container that holds the reference sources: the reference's BCL::gemm(CSRMatrix...), spmm_wrapper,
+SparseHashAccumulator::get_matrix and CSRMatrix::read_MatrixMarket are compiled from those sources into oracle/_ref (make -C oracle ref) and called
 on small seeded inputs. Each fixture stores the inputs and the reference's outputs (.npz, no pickles):
 
   python tests/golden/make_golden.py
@@ -65,6 +65,37 @@ def spgemm_cases():
     return cases
 
 
+def mm_cases():
+    """Small MatrixMarket files (text) covering the reader's cases: comments, repeated entries (first wins),
+    symmetric mirroring, pattern and integer fields, 0-based indices, empty rows."""
+    rng = np.random.default_rng(40)
+
+    def text(m, n, ents, field, sym, one):
+        b = 1 if one else 0
+        out = [f"%%MatrixMarket matrix coordinate {field} {sym}", "% written by make_golden.py", f"{m} {n} {len(ents)}"]
+        for i, j, v in ents:
+            out.append(f"{i + b} {j + b}" if field == "pattern" else
+                       f"{i + b} {j + b} {int(v)}" if field == "integer" else f"{i + b} {j + b} {v:.9g}")
+        return ("\n".join(out) + "\n").encode()
+
+    def ents(m, n, count, lower):
+        ii, jj = rng.integers(0, m, count), rng.integers(0, n, count)
+        if lower:
+            ii, jj = np.maximum(ii, jj), np.minimum(ii, jj)
+        e = list(zip(ii.tolist(), jj.tolist(), rng.standard_normal(count).tolist()))
+        e += [(e[q][0], e[q][1], float(rng.standard_normal())) for q in rng.integers(0, count, count // 5)]
+        return e
+
+    return {
+        "general": (text(60, 40, ents(60, 40, 300, False), "real", "general", True), True),
+        "symmetric": (text(50, 50, ents(50, 50, 250, True), "real", "symmetric", True), True),
+        "pattern": (text(30, 70, ents(30, 70, 200, False), "pattern", "general", True), True),
+        "integer": (text(40, 40, [(i, j, float(rng.integers(-9, 9))) for i, j, _ in ents(40, 40, 150, True)],
+                         "integer", "symmetric", True), True),
+        "zero_based": (text(25, 35, ents(25, 35, 120, False), "real", "general", False), False),
+    }
+
+
 def main():
     if not O.have_ref():
         raise SystemExit("oracle/_ref/libbclref.so is missing: run `make -C oracle ref` next to the reference tree")
@@ -91,6 +122,18 @@ def main():
     for t, tt in enumerate(tiles):
         d.update(pack(f"t{t}", tt))
     out["accumulate4"] = d
+    # CSRMatrix::read_MatrixMarket on small files
+    import tempfile
+    mm = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for nm, (txt, one) in mm_cases().items():
+            path = os.path.join(tmp, nm + ".mtx")
+            with open(path, "wb") as f:
+                f.write(txt)
+            c = O.ref_mm_read(path, one_indexed=one)
+            mm.update({f"{nm}_text": np.frombuffer(txt, np.uint8), f"{nm}_one": np.array([int(one)], np.int64),
+                       **pack(nm, c)})
+    out["mm_read"] = mm
     for name, d in out.items():
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
         print(name, {k: v.shape for k, v in d.items() if k.endswith("col_idx")})
