@@ -58,6 +58,7 @@ SIGNATURES = [
     ("bclx_stage_execute", _I32, [_P]),
     ("bclx_stage_result", _I32, [_P, _I32, ctypes.POINTER(bclx_csr)]),
     ("bclx_stage_stats", _I32, [_P, _P, _I32]),
+    ("bclx_stage_predictions", _I32, [_P, _I64, _P, _P, _P, _P, _P, ctypes.POINTER(_I64)]),
     ("bclx_stage_plan_destroy", _I32, [_P]),
     ("bclx_row_features", _I32, [_P, ctypes.POINTER(bclx_csr), ctypes.POINTER(bclx_csr), _P, _P]),
     ("bclx_mlp_forward", _I32, [_P, _I64, _P, _P, _P]),
@@ -66,6 +67,8 @@ SIGNATURES = [
     ("bclx_spmm", _I32, [_P, ctypes.POINTER(bclx_csr), _P, _I32, _I64, _P, _I64, _I64, _F32, _F32]),
     ("bclx_gemm", _I32, [_P, _I32, _I32, _I64, _I64, _I64, _F32, _P, _I64, _P, _I64, _F32, _P, _I64, _I32]),
     ("bclx_gemm_f64_ref", _I32, [_P, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64]),
+    ("bclx_csr_split_grid", _I32, [_P, ctypes.POINTER(bclx_csr), _I64, _I64, _I32, ctypes.POINTER(bclx_csr)]),
+    ("bclx_dense_binary_op", _I32, [_P, _I32, _I32, _I64, _I64, _P, _I64, _P, _I64, _P, _I64]),
     ("bclx_mm_info", _I32, [ctypes.c_char_p, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     ("bclx_mm_read", _I32, [ctypes.c_char_p, _I32, _I32, _I32, ctypes.POINTER(bclx_csr)]),
     ("bclx_mm_write", _I32, [ctypes.c_char_p, ctypes.POINTER(bclx_csr)]),

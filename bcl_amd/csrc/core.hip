@@ -1,4 +1,6 @@
 // Handle lifecycle, error reporting and version of the bclx C ABI.
+#include <memory>
+
 #include "internal.hpp"
 
 namespace bclx {
@@ -41,7 +43,7 @@ bclx_status bclx_create(int device, void* stream, bclx_handle** out) {
     BCLX_REQUIRE(device >= 0 && device < ndev, BCLX_ERR_INVALID,
                  "device " + std::to_string(device) + " not present (" + std::to_string(ndev) + " devices)");
     BCLX_HIP(hipSetDevice(device));
-    auto h = new bclx_handle();
+    std::unique_ptr<bclx_handle> h(new bclx_handle());
     h->device = device;
     h->stream = static_cast<hipStream_t>(stream);
     int cus = 0, lds = 0;
@@ -49,7 +51,7 @@ bclx_status bclx_create(int device, void* stream, bclx_handle** out) {
     BCLX_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     h->num_cus = cus > 0 ? cus : 256;
     h->lds_per_block = lds > 0 ? (size_t)lds : 65536;
-    *out = h;
+    *out = h.release();
   });
 }
 

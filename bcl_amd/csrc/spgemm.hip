@@ -488,10 +488,13 @@ struct TwoPhasePlan : PlanBase {
   void numeric(bclx_csr* C) override {
     BCLX_REQUIRE(C && C->row_ptr, BCLX_ERR_INVALID, "C or C->row_ptr is null");
     BCLX_REQUIRE(C->m == tp.m, BCLX_ERR_INVALID, "C has the wrong number of rows");
-    BCLX_REQUIRE(tp.nnz_c == 0 || (C->col_idx && C->vals), BCLX_ERR_INVALID, "C->col_idx / C->vals are null");
     BCLX_REQUIRE(C->ptr_width == 4 || C->ptr_width == 8, BCLX_ERR_INVALID, "C->ptr_width must be 4 or 8");
     BCLX_REQUIRE(C->dtype == (std::is_same<T, float>::value ? BCLX_F32 : BCLX_F64), BCLX_ERR_INVALID,
                  "C dtype must match A and B");
+    // a result that cannot be indexed by the requested row_ptr is refused before its arrays are looked at
+    BCLX_REQUIRE(tp.nnz_c < 0 || C->ptr_width == 8 || tp.nnz_c <= INT32_MAX, BCLX_ERR_OVERFLOW,
+                 "nnz(C) = " + std::to_string(tp.nnz_c) + " does not fit an int32 row_ptr; request ptr_width 8");
+    BCLX_REQUIRE(tp.nnz_c == 0 || (C->col_idx && C->vals), BCLX_ERR_INVALID, "C->col_idx / C->vals are null");
     tp.numeric(C->col_idx, static_cast<T*>(C->vals), C->nnz, C->row_ptr, C->ptr_width);
     C->nnz = tp.nnz_c;
   }

@@ -106,6 +106,8 @@ struct RowsDev {
   int32_t* pred_count;       // rows whose capacity the predictor sets (learned path)
   int32_t* long_count;       // rows of more than 32 A entries, listed by the analysis in tmp_items
   int32_t* pred_items;
+  float* pred_val;             // the network's raw prediction per listed row (learned path; pred_val_cap entries)
+  int32_t pred_val_cap;
   unsigned long long* cursor;  // bump allocator for rows placed outside their predicted slot
   unsigned long long* heavy_queue;  // next heavy row to take (rows are handed out largest first)
   int32_t* tmp_items;               // scratch list for ordering the heavy bin

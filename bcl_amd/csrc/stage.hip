@@ -41,6 +41,7 @@ __global__ __launch_bounds__(kPredictBlock) void k_predict(RowsDev<T, RP> R, con
     row_features<RP>(tile.a_rp, tile.a_col, tile.b_rp, tile.b_col, i, tile.n, 0, tile.avg_a, tile.avg_b, prod, f);
     const float y = mlp_forward_row(P, f);
     const double pred = exp2((double)fminf(fmaxf(y, 0.f), 40.f)) - 1.0;
+    if (idx < R.pred_val_cap) R.pred_val[idx] = (float)pred;
     const long long c = (long long)ceil(pred * (1.0 + (double)margin)) + slack;
     long long cap = c < ub ? c : ub;
     if (cap < 1) cap = 1;
@@ -116,11 +117,23 @@ __global__ __launch_bounds__(kRowBlock) void k_tile_rowptr(RowsDev<T, RP> R, con
   if (tile.m == 0 && i == 0) rp[fr + t] = 0;
 }
 
+// Capacity and exact count of the listed rows: out[2 q] = cap, out[2 q + 1] = nnz of flat row pred_items[q].
+template <typename T, typename RP>
+__global__ void k_gather_pred(RowsDev<T, RP> R, int cnt, int32_t* out) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+    const int fr = R.pred_items[q];
+    out[2 * q] = R.cap[fr];
+    out[2 * q + 1] = R.nnz[fr];
+  }
+}
+
 struct StageBase {
   virtual ~StageBase() = default;
   virtual void execute() = 0;
   virtual void result(int p, bclx_csr* C) = 0;
   virtual void stats(int64_t* out, int n) = 0;
+  virtual void predictions(int64_t max, int32_t* product, int32_t* row, float* pred, int32_t* cap, int32_t* nnz,
+                           int64_t* count) = 0;
 };
 
 template <typename T, typename RP>
@@ -137,7 +150,7 @@ struct StagePlan : StageBase {
   std::vector<int64_t> c_len;  // nnz of each product in the last execution
   // modes 1 / 2
   FlatRows<T, RP> F;
-  DevBuf params, arena_col, arena_val, final_col, final_val, final_off, tile_rp, chunk_row;
+  DevBuf params, arena_col, arena_val, final_col, final_val, final_off, tile_rp, chunk_row, pred_vals;
   int64_t arena_cap = 0, final_cap = 0, main_cap = 0;
   int32_t counts1[kNumBins] = {0}, counts2[kNumBins] = {0};
   std::vector<hipStream_t> side;
@@ -326,6 +339,15 @@ struct StagePlan : StageBase {
       fill_i32(s, F.R.nnz, F.nflat, 0);
       predict(s, true);
       setup_heavy(s);
+      if (opts.h_mlp_params) {  // room for the raw predictions of the listed rows (read by bclx_stage_predictions)
+        int32_t npred = 0;
+        BCLX_HIP(hipMemcpyAsync(&npred, F.R.pred_count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        BCLX_HIP(hipStreamSynchronize(s));
+        const int64_t room = std::min<int64_t>(F.nflat, (int64_t)npred + npred / 4 + 1024);
+        pred_vals.alloc(sizeof(float) * (size_t)room);
+        F.R.pred_val = pred_vals.as<float>();
+        F.R.pred_val_cap = (int32_t)room;
+      }
       F.scan(s, F.R.cap, F.R.off, 0);
       int64_t tot_cap = 0;
       BCLX_HIP(hipMemcpyAsync(&tot_cap, F.scan_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -476,6 +498,44 @@ struct StagePlan : StageBase {
     C->vals = final_val.as<T>() + lo;
   }
 
+  void predictions(int64_t max, int32_t* product, int32_t* row, float* pred, int32_t* cap, int32_t* nnz,
+                   int64_t* count) override {
+    BCLX_REQUIRE(mode == 1 && opts.h_mlp_params, BCLX_ERR_INVALID,
+                 "predictions exist only in a learned plan (mode 1) made with network weights");
+    BCLX_HIP(hipStreamSynchronize(h->stream));
+    int32_t cnt = 0;
+    BCLX_HIP(hipMemcpy(&cnt, F.R.pred_count, sizeof(int32_t), hipMemcpyDeviceToHost));
+    cnt = std::min<int32_t>(cnt, F.R.pred_val_cap);
+    *count = cnt;
+    const int64_t k = std::min<int64_t>(cnt, max);
+    if (k <= 0) return;
+    std::vector<int32_t> items((size_t)k), capnnz(2 * (size_t)k);
+    DevBuf tmp(sizeof(int32_t) * 2 * (size_t)cnt);
+    hipLaunchKernelGGL((k_gather_pred<T, RP>), dim3(std::max<unsigned>(1, std::min<unsigned>(div_up(cnt, 256), 4096))),
+                       dim3(256), 0, h->stream, F.R, cnt, tmp.as<int32_t>());
+    BCLX_HIP(hipGetLastError());
+    BCLX_HIP(hipMemcpyAsync(items.data(), F.R.pred_items, sizeof(int32_t) * (size_t)k, hipMemcpyDeviceToHost,
+                            h->stream));
+    BCLX_HIP(hipMemcpyAsync(capnnz.data(), tmp.p, sizeof(int32_t) * 2 * (size_t)k, hipMemcpyDeviceToHost, h->stream));
+    if (pred)
+      BCLX_HIP(hipMemcpyAsync(pred, F.R.pred_val, sizeof(float) * (size_t)k, hipMemcpyDeviceToHost, h->stream));
+    BCLX_HIP(hipStreamSynchronize(h->stream));
+    for (int64_t q = 0; q < k; q++) {
+      const int64_t fr = items[(size_t)q];
+      int t = 0;  // the product holding flat row fr: the last with flat0 <= fr
+      int lo = 0, hi = npairs - 1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) / 2;
+        if (F.tiles[mid].flat0 <= fr) { t = mid; lo = mid + 1; }
+        else hi = mid - 1;
+      }
+      if (product) product[q] = t;
+      if (row) row[q] = (int32_t)(fr - F.tiles[t].flat0);
+      if (cap) cap[q] = capnnz[2 * (size_t)q];
+      if (nnz) nnz[q] = capnnz[2 * (size_t)q + 1];
+    }
+  }
+
   void stats(int64_t* out, int n) override {
     BCLX_HIP(hipStreamSynchronize(h->stream));
     int64_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -567,6 +627,15 @@ bclx_status bclx_stage_stats(bclx_stage_plan* plan, int64_t* h_stats, int nstats
     BCLX_REQUIRE(plan && h_stats, BCLX_ERR_INVALID, "null argument");
     set_device(plan->h);
     plan->impl->stats(h_stats, nstats);
+  });
+}
+
+bclx_status bclx_stage_predictions(bclx_stage_plan* plan, int64_t max, int32_t* h_product, int32_t* h_row,
+                                   float* h_pred, int32_t* h_cap, int32_t* h_nnz, int64_t* count) {
+  return guarded([&] {
+    BCLX_REQUIRE(plan && count && max >= 0, BCLX_ERR_INVALID, "bad argument");
+    set_device(plan->h);
+    plan->impl->predictions(max, h_product, h_row, h_pred, h_cap, h_nnz, count);
   });
 }
 

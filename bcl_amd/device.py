@@ -144,6 +144,41 @@ def accumulate(h: Handle, tiles: list, ptr_dtype=torch.int64) -> DeviceCSR:
     return C
 
 
+def split_grid(h: Handle, A: DeviceCSR, tile_m: int, tile_n: int, ptr_dtype=torch.int32) -> dict:
+    """The tiles of A on a grid of tile_m x tile_n tiles, cut on the GPU (SPMatrix's constructor, through
+    CSRMatrix::get_slice_impl_): {(ti, tj): DeviceCSR}. Two calls through the C ABI: sizes, then the copy."""
+    gm = max(1, -(-A.m // tile_m))
+    gn = max(1, -(-A.n // tile_n))
+    arr = (bclx_csr * (gm * gn))()
+    da = A.desc()
+    call("bclx_csr_split_grid", h.h, ctypes.byref(da), tile_m, tile_n, gm * gn, arr)
+    dev = A.vals.device
+    out = {}
+    for t in range(gm * gn):
+        d = arr[t]
+        T = DeviceCSR(d.m, d.n, torch.empty(d.m + 1, dtype=ptr_dtype, device=dev),
+                      torch.empty(d.nnz, dtype=torch.int32, device=dev), torch.empty(d.nnz, dtype=A.vals.dtype,
+                                                                                      device=dev))
+        arr[t] = T.desc()
+        out[(t // gn, t % gn)] = T
+    call("bclx_csr_split_grid", h.h, ctypes.byref(da), tile_m, tile_n, gm * gn, arr)
+    return out
+
+
+OP_ADD, OP_SUB, OP_MUL, OP_DIV = 0, 1, 2, 3
+
+
+def dense_binary_op(h: Handle, op: int, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor | None = None) -> torch.Tensor:
+    """C = A op B elementwise on row-major tiles (DMatrix's +, -, *, /); C = A for the in-place form (+=)."""
+    C = A if C is None else C
+    assert A.dim() == B.dim() == C.dim() == 2 and A.shape == B.shape == C.shape and A.dtype == B.dtype == C.dtype
+    assert A.dtype in _DT and all(t.stride(1) == 1 or t.shape[1] <= 1 for t in (A, B, C))
+    m, n = A.shape
+    ld = lambda t: max(t.stride(0), n)
+    call("bclx_dense_binary_op", h.h, op, _DT[A.dtype], m, n, _ptr(A), ld(A), _ptr(B), ld(B), _ptr(C), ld(C))
+    return C
+
+
 GEMM_FP32, GEMM_BF16, GEMM_SPLIT_BF16 = 0, 1, 2
 
 
@@ -177,4 +212,5 @@ def version() -> str:
     return lib().bclx_version().decode()
 
 
-__all__ = ["Handle", "DeviceCSR", "spgemm", "spmm", "accumulate", "gemm", "gemm_f64_ref", "version", "_lib"]
+__all__ = ["Handle", "DeviceCSR", "spgemm", "spmm", "accumulate", "split_grid", "dense_binary_op", "gemm",
+           "gemm_f64_ref", "version", "_lib"]

@@ -59,6 +59,18 @@ class StagePlan:
         call("bclx_stage_stats", self.p, out.ctypes.data_as(ctypes.c_void_p), len(STAT_NAMES))
         return dict(zip(STAT_NAMES, [int(x) for x in out]))
 
+    def predictions(self) -> dict:
+        """The last execution's network predictions: arrays product, row, pred (float), cap, nnz per sized row."""
+        cnt = ctypes.c_int64()
+        call("bclx_stage_predictions", self.p, 0, None, None, None, None, None, ctypes.byref(cnt))
+        k = cnt.value
+        out = {"product": np.zeros(k, np.int32), "row": np.zeros(k, np.int32), "pred": np.zeros(k, np.float32),
+               "cap": np.zeros(k, np.int32), "nnz": np.zeros(k, np.int32)}
+        ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        call("bclx_stage_predictions", self.p, k, ptr(out["product"]), ptr(out["row"]), ptr(out["pred"]),
+             ptr(out["cap"]), ptr(out["nnz"]), ctypes.byref(cnt))
+        return out
+
     def result_desc(self, p: int) -> bclx_csr:
         d = bclx_csr()
         call("bclx_stage_result", self.p, p, ctypes.byref(d))

@@ -146,6 +146,18 @@ def bench_stage(args):
     st = learned.stats()
     log("learned stats", st, "per-step ms", [round(x, 2) for x in per_l])
 
+    pr = learned.predictions()
+    pred_stats = None
+    if pr["pred"].size:
+        ok = pr["nnz"] > 0
+        rel = np.abs(pr["pred"][ok].astype(np.float64) - pr["nnz"][ok]) / pr["nnz"][ok]
+        pred_stats = {"rows": int(pr["pred"].size), "rel_err_median": round(float(np.median(rel)), 5),
+                      "rel_err_p99": round(float(np.quantile(rel, 0.99)), 5),
+                      "rel_err_max": round(float(rel.max()), 5),
+                      "capacity_below_exact": int((pr["cap"] < pr["nnz"]).sum()),
+                      "capacity_over_exact": round(float(pr["cap"].sum() / max(1, pr["nnz"].sum())), 5)}
+    learned.close()
+
     res = {}
     if not args.skip_plain:
         t = time.time()
@@ -157,11 +169,20 @@ def bench_stage(args):
         log("plain per-step ms", [round(x, 2) for x in per_p])
         plain.close()
         res["plain"] = ms_p / args.steps
-    if args.exact:
+    if not args.skip_exact:
         ex = StagePlan(h, A, B, mode=BATCHED_EXACT, tuning=tune)
         ms_e, per_e = time_steps(ex.execute, args.steps, args.warmup, stream)
+        assert ex.stats()["nnz"] == st["nnz"]
         res["batched_exact"] = ms_e / args.steps
         ex.close()
+        # the same learned pipeline without the network: every row keeps its exact upper bound min(products, n)
+        bd = StagePlan(h, A, B, mode=LEARNED, params=None, margin=margin, slack=slack, tuning=tune)
+        ms_b, per_b = time_steps(bd.execute, args.steps, args.warmup, stream)
+        st_b = bd.stats()
+        assert st_b["nnz"] == st["nnz"]
+        res["bound"] = (ms_b / args.steps, st_b)
+        bd.close()
+        log("exact / bound per-step ms", round(res["batched_exact"], 3), round(res["bound"][0], 3))
 
     products = st["products"]
     flops = 2.0 * products
@@ -211,14 +232,61 @@ def bench_stage(args):
     if "batched_exact" in res:
         out["batched_exact_gpu"] = {"ms_per_step": round(res["batched_exact"], 4),
                                     "value": round(flops / (res["batched_exact"] * 1e-3) / 1e9, 3)}
-    learned.close()
+        ms_b, st_b = res["bound"]
+        # what the trained network changes at this stage: the same learned pipeline with and without it
+        out["predictor"] = {
+            "trained_ms": round(ms_step, 4), "without_network_ms": round(ms_b, 4),
+            "time_gain": round(ms_b / ms_step, 4),
+            "slot_capacity_trained": st["capacity"], "slot_capacity_without_network": st_b["capacity"],
+            "capacity_saved_entries": int(st_b["capacity"] - st["capacity"]),
+            "rows_sized_by_network": st["predicted_rows"], "overflow_rows": st["overflow_rows"],
+            "prediction": pred_stats,
+            "note": "rows the network sizes are the heavy rows (more products than the sort bins take); they count "
+                    "exactly before writing, so a short slot costs a bump allocation, not a redo"}
     if not args.no_cpu:
         t = time.time()
         cb, cbp = cpu_baseline_stage(pairs, args.cpu_products)
         out["cpu_baseline"] = cb
         out["cpu_baseline_parallel"] = cbp
         log(f"cpu baseline ({time.time() - t:.1f}s)")
+    del A, B, pairs
+    torch.cuda.empty_cache()
+    if not args.headline_only and args.extra_stage_k >= 0:
+        out[f"stage_k{args.extra_stage_k}"] = bench_other_stage(args, h, tiles, args.extra_stage_k, params, margin,
+                                                                slack, tune)
     return out
+
+
+def bench_other_stage(args, h, tiles, k, params, margin, slack, tune):
+    """Another SUMMA stage of the same grid (learned against plain), e.g. the heavier k = 2: 12.5 G products."""
+    import torch
+    from bcl_amd.stage import LEARNED, PLAIN, StagePlan, stage_tiles
+    A, B, pairs = stage_tiles(tiles, tiles, args.grid, k)
+    steps, warm = max(2, args.steps // 2), 1
+    t = time.time()
+    plan = StagePlan(h, A, B, mode=LEARNED, params=params, margin=margin, slack=slack, tuning=tune)
+    ms_l, _ = time_steps(plan.execute, steps, warm, h.stream)
+    st = plan.stats()
+    plan.close()
+    torch.cuda.empty_cache()
+    plain = StagePlan(h, A, B, mode=PLAIN, tuning=tune)
+    ms_p, _ = time_steps(plain.execute, steps, warm, h.stream)
+    st_p = plain.stats()
+    plain.close()
+    assert st_p["nnz"] == st["nnz"] and st_p["products"] == st["products"], (st_p, st)
+    log(f"stage k={k}: learned {ms_l / steps:.2f} ms, plain {ms_p / steps:.2f} ms ({time.time() - t:.1f}s)")
+    distinct_a = {id(x): x for x in A}
+    distinct_b = {id(x): x for x in B}
+    alg = sum(csr_bytes(x.m, x.nnz) for x in distinct_a.values()) + \
+        sum(csr_bytes(x.m, x.nnz) for x in distinct_b.values()) + (st["rows"] + len(pairs)) * 8 + st["nnz"] * 8
+    ms = ms_l / steps
+    del A, B
+    torch.cuda.empty_cache()
+    return {"workload": f"rmat-s{args.scale}-ef16-grid{args.grid}-stage{k}", "steps": steps, "products": st["products"],
+            "nnz_c": st["nnz"], "learned_ms": round(ms, 3), "value": round(2 * st["products"] / (ms * 1e-3) / 1e9, 3),
+            "unit": "GFLOP/s", "plain_ms": round(ms_p / steps, 3), "speedup_vs_plain": round(ms_p / ms_l, 4),
+            "hbm_gbs": round(alg / (ms * 1e-3) / 1e9, 2), "roofline_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "algorithmic_bytes": int(alg), "overflow_rows": st["overflow_rows"], "predicted_rows": st["predicted_rows"]}
 
 
 def _ref_or_oracle():
@@ -291,10 +359,17 @@ def bench_spmm(args):
         ref = O.spmm_f64(sub, bh)
         got = C.cpu().numpy()[rows]
         err = float(np.abs(got - ref).max() / np.abs(ref).max())
+        absum = O.spmm_f64(CSR(sub.m, sub.n, sub.row_ptr, sub.col_idx, np.abs(sub.vals)), np.abs(bh))
+        e_s = (np.abs(got - ref) / np.maximum(absum, 1e-300)).ravel()
+        big = (np.abs(ref) >= 0.01 * absum).ravel()
+        e_r = (np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)).ravel()[big]
         b_bytes = a.n * n * (4 if name == "fp32" else 2)
         alg = csr_bytes(a.m, a.nnz) + b_bytes + 2 * a.m * n * 4
         out[name] = {"ms": round(ms_step, 4), "gflops": round(2 * a.nnz * n / (ms_step * 1e-3) / 1e9, 2),
                      "hbm_gbs": round(alg / (ms_step * 1e-3) / 1e9, 2), "max_err_vs_fp64_rel_to_max": err,
+                     "err_over_sum_abs_median": float(np.median(e_s)), "err_over_sum_abs_p99": float(np.quantile(e_s, 0.99)),
+                     "rel_err_median": float(np.median(e_r)), "rel_err_p99": float(np.quantile(e_r, 0.99)),
+                     "error_sample": "2048 random rows x 128 columns against the fp32-B product in FP64",
                      "algorithmic_bytes": alg}
     # reference CPU spmm_wrapper on a row sample
     rs = min(a.m, 65536)
@@ -336,16 +411,27 @@ def bench_gemm(args):
     peak = {GEMM_FP32: 157.3, GEMM_BF16: 2500.0, GEMM_SPLIT_BF16: 2500.0}
     issued = {GEMM_FP32: 1, GEMM_BF16: 1, GEMM_SPLIT_BF16: 3}
     res = {}
+    S = gemm_f64_ref(h, A.abs(), B.abs())  # sum_k |a_ik| |b_kj|: the scale of each entry's rounding error
+    pick = torch.randint(0, N * N, (1 << 20,), device="cuda", generator=torch.Generator(device="cuda").manual_seed(5))
     for mode, name in ((GEMM_FP32, "fp32"), (GEMM_BF16, "bf16"), (GEMM_SPLIT_BF16, "split_bf16_3")):
         C = torch.zeros((N, N), device="cuda")
         ms, _ = time_steps(lambda: gemm(h, A, B, C, mode=mode, beta=0.0), args.steps, args.warmup, h.stream)
         ms_step = ms / args.steps
         torch.cuda.synchronize()
-        err = float(((C.double() - C64).abs().max() / C64.abs().max()).item())
+        d = (C.double() - C64).abs()
+        err = float((d.max() / C64.abs().max()).item())
+        e_s = (d / S).flatten()[pick]
+        big = (C64.abs() >= 0.01 * S).flatten()[pick]  # entries that are not cancellation residues
+        e_r = ((d / C64.abs().clamp_min(1e-300)).flatten()[pick])[big]
+        q = lambda x, p: float(torch.quantile(x, p).item())
         useful = 2.0 * N ** 3 / (ms_step * 1e-3) / 1e12
         res[name] = {"ms": round(ms_step, 3), "useful_tflops": round(useful, 2),
                      "mfma_util": round(useful * issued[mode] / peak[mode], 4),
-                     "max_abs_err_rel_to_max_c": err}
+                     "max_abs_err_rel_to_max_c": err,
+                     "err_over_sum_abs_median": q(e_s, 0.5), "err_over_sum_abs_p99": q(e_s, 0.99),
+                     "rel_err_median": q(e_r, 0.5), "rel_err_p99": q(e_r, 0.99),
+                     "error_sample": "2^20 random entries of C; rel_err over entries with |C| >= 0.01 sum|a||b|"}
+        del C, d
     f = res["bf16"]
     # CPU: the oracle's float64 GEMM on 16 rows (labelled)
     t = time.perf_counter()
@@ -420,9 +506,12 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--grid", type=int, default=8)
     ap.add_argument("--stage-k", type=int, default=3)
+    ap.add_argument("--extra-stage-k", type=int, default=2, help="a second stage timed beside the headline (-1: none)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--skip-plain", action="store_true")
-    ap.add_argument("--exact", action="store_true", help="also time the batched exact (symbolic) pipeline")
+    ap.add_argument("--skip-exact", action="store_true", help="skip the batched exact and without-network runs")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the stage workload (no extra stage, config 1, SpMM, GEMM or Laplacian keys)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-products", type=int, default=25_000_000)
     args = ap.parse_args()
@@ -430,9 +519,18 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only: the stage's products run on one MI355X; "
                                    "multi-GPU SUMMA (tile broadcasts over RCCL/xGMI) is a design, not built"}))
         sys.exit(2)
-    fn = {"stage": bench_stage, "config1": bench_config1, "spmm": bench_spmm, "gemm": bench_gemm,
-          "laplacian": bench_laplacian}[args.workload]
-    out = fn(args)
+    fns = {"stage": bench_stage, "config1": bench_config1, "spmm": bench_spmm, "gemm": bench_gemm,
+           "laplacian": bench_laplacian}
+    out = fns[args.workload](args)
+    if args.workload == "stage" and not args.headline_only:
+        # every other configuration of the benchmark, each with its own roofline and CPU baseline, as extra keys
+        for name in ("config1", "spmm", "gemm", "laplacian"):
+            t = time.time()
+            sub = fns[name](args)
+            for key in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
+                sub.pop(key, None)
+            out[name] = sub
+            log(f"{name} done ({time.time() - t:.1f}s)")
     print(json.dumps(out), flush=True)
 
 

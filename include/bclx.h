@@ -139,6 +139,12 @@ BCLX_API bclx_status bclx_stage_result(bclx_stage_plan* plan, int p, bclx_csr* C
  * [5] rows, [6] heavy rows, [7] sorted (ESC) rows. Fails with BCLX_ERR_OVERFLOW, as bclx_stage_result does, when the
  * last execution refused rows for lack of room. */
 BCLX_API bclx_status bclx_stage_stats(bclx_stage_plan* plan, int64_t* h_stats, int nstats);
+/* The row predictions of the last execution of a learned plan (mode 1, made with network weights): for each of the
+ * first max rows the network sized, the product p, the row within it, the network's predicted entry count, the
+ * capacity that row was given and its exact entry count (any array may be NULL); *count = rows the network sized.
+ * Diagnostic: it reads back what the graph computed. */
+BCLX_API bclx_status bclx_stage_predictions(bclx_stage_plan* plan, int64_t max, int32_t* h_product, int32_t* h_row,
+                                            float* h_pred, int32_t* h_cap, int32_t* h_nnz, int64_t* count);
 BCLX_API bclx_status bclx_stage_plan_destroy(bclx_stage_plan* plan);
 
 /* ---- learned row-work predictor (MLP 16 -> 64 -> 64 -> 1, ReLU) -------------------------------------------
@@ -173,6 +179,24 @@ BCLX_API bclx_status bclx_gemm(bclx_handle* h, int32_t trans_a, int32_t trans_b,
 /* FP64 dense GEMM C = A * B + C (fp32 inputs widened exactly, fp64 C): the GPU error oracle for large sizes. */
 BCLX_API bclx_status bclx_gemm_f64_ref(bclx_handle* h, int64_t m, int64_t n, int64_t k, const float* d_A,
                                        int64_t lda, const float* d_B, int64_t ldb, double* d_C, int64_t ldc);
+
+/* ---- tile ingest and dense tile arithmetic ----------------------------------------------------------------
+ * bclx_csr_split_grid: the tiles of A (device CSR, columns sorted within rows) on a grid of tile_m x tile_n tiles,
+ * tile (ti, tj) = rows [ti tile_m, (ti+1) tile_m) and columns [tj tile_n, (tj+1) tile_n) with columns shifted to 0,
+ * as SPMatrix's constructor cuts a matrix into tiles (SPMatrix::init_, bcl/containers/SPMatrix.hpp:212-266, through
+ * CSRMatrix::get_slice_impl_, bcl/containers/sequential/CSRMatrix.hpp:238-276). tiles holds ntiles =
+ * ceil(m / tile_m) * ceil(n / tile_n) descriptors, row-major over the grid. Two calls: with every tiles[t].row_ptr
+ * NULL, each descriptor gets m, n, nnz and dtype; then, with tiles[t].row_ptr (m + 1 of tiles[t].ptr_width),
+ * col_idx and vals (nnz each) allocated by the caller, the tiles are written. At most 256 tile columns. Columns that
+ * are not sorted or not below A->n fail with BCLX_ERR_INVALID. */
+BCLX_API bclx_status bclx_csr_split_grid(bclx_handle* h, const bclx_csr* A, int64_t tile_m, int64_t tile_n,
+                                         int32_t ntiles, bclx_csr* tiles);
+/* C = A op B elementwise on m x n row-major tiles (op 0 +, 1 -, 2 *, 3 /; dtype BCLX_F32 or BCLX_F64), each with its
+ * own leading dimension; C may be A (in place, with ldc == lda). DMatrix::binary_op / binary_op_inplace and its
+ * operators +, -, *, / and += (bcl/containers/DMatrix.hpp:319-386) on one tile. */
+BCLX_API bclx_status bclx_dense_binary_op(bclx_handle* h, int32_t op, int32_t dtype, int64_t m, int64_t n,
+                                          const void* d_A, int64_t lda, const void* d_B, int64_t ldb, void* d_C,
+                                          int64_t ldc);
 
 /* ---- MatrixMarket files (host memory, no GPU needed) -------------------------------------------------------
  * bclx_mm_info: m, n and the stored entry count from the size line

@@ -1,22 +1,34 @@
 // C++ host layer over the bclx C ABI: the reference's local tile multiply interfaces with the same names, argument
 // meaning and error behaviour (std::runtime_error), running on the GPU.
 //
-//   bclx::CSRMatrix<T, I>        BCL::CSRMatrix            bcl/containers/sequential/CSRMatrix.hpp:32-366
+//   bclx::CSRMatrix<T, I, A>     BCL::CSRMatrix            bcl/containers/sequential/CSRMatrix.hpp:32-366
 //                                (same public members m_, n_, nnz_, vals_, row_ptr_, col_ind_; binary file format
 //                                of read_Binary / write_Binary, CSRMatrix.hpp:488-530; get_slice_impl_ :238-276)
 //   bclx::gemm(a, b, c)          BCL::gemm(CSRMatrix, CSRMatrix, CSRMatrix)  bcl/containers/algorithms/spgemm.hpp:40-76
 //                                (c += a * b; exact-zero sums are kept)
-//   a.dot(b)                     CSRMatrix::dot            CSRMatrix.hpp:229-235
+//   a.dot(b), bclx::dot(a, b)    CSRMatrix::dot            CSRMatrix.hpp:229-235
+//   (bclx::dot, bclx::gemm, bclx::split_grid and bclx::SparseHashAccumulator also take BCL::CSRMatrix<T, I, A>
+//   itself, or any type with its public members and constructor, and return that type)
 //   bclx::spmm_wrapper(...)      BCL::spmm_wrapper         spgemm.hpp:140-149 (c += a * b, row-major, ldb / ldc)
 //   bclx::cblas_gemm_wrapper_()  BCL::experimental::cblas_gemm_wrapper_  cblas_wrapper.hpp:16-40
 //   bclx::SparseHashAccumulator  BCL::SparseHashAccumulator  SparseHashAccumulator.hpp:86-183
+//   CSRMatrix::read_MatrixMarket / write_MatrixMarket, CSRMatrix(fname, format), matrix_io::matrix_info
+//                                CSRMatrix.hpp:191-224, :369-411, :532-548; matrix_io.hpp:10-98, :221-357
+//                                (read in linear time by libbclx; the reference's reader is quadratic)
+//   bclx::CSRMatrixMemoryMapped  BCL::CSRMatrixMemoryMapped  CSRMatrixMemoryMapped.hpp:55-145
+//   bclx::split_grid(a, tm, tn)  the tile cut of SPMatrix's constructor (SPMatrix.hpp:212-266), on the GPU
+//   bclx::tile_binary_op(...)    DMatrix::binary_op / binary_op_inplace on one tile (DMatrix.hpp:319-386), on the GPU
 //
 // Host containers hold std::vector data like the reference; each call copies operands to HBM, runs the HIP kernels
 // through the C ABI on a process-wide handle (device 0 unless bclx::set_device is called first, default stream) and
 // copies results back. Values of sparse products are accumulated in float64 and rounded once.
 #pragma once
 
+#include <fcntl.h>
 #include <hip/hip_runtime_api.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <array>
@@ -32,6 +44,9 @@
 #include "../bclx.h"
 
 namespace bclx {
+
+// File formats of CSRMatrix(fname, format) (BCL::FileFormat, bcl/containers/sequential/matrix_io.hpp:10-15).
+enum FileFormat { MatrixMarket, MatrixMarketZeroIndexed, Binary, Unknown };
 
 // CBLAS enumerator values (CBLAS_ORDER / CBLAS_TRANSPOSE), so CBLAS constants cast to int can be passed through.
 enum Layout { RowMajor = 101, ColMajor = 102 };
@@ -108,24 +123,31 @@ inline void set_device(int device) {
 }
 inline void set_gemm_mode(GemmMode m) { detail::ctx().gemm_mode = m; }
 
-template <typename T, typename I = int>
+// BCL::CSRMatrix<T, I, Allocator> (CSRMatrix.hpp:32-53): the same template parameters, member names and layout; the
+// index arrays use the allocator rebound to I, as the reference's IAllocator.
+template <typename T, typename I = int, typename Allocator = std::allocator<T>>
 struct CSRMatrix {
   using size_type = size_t;
   using value_type = T;
+  using index_t = I;
   using index_type = I;
+  using allocator_traits = std::allocator_traits<Allocator>;
+  using IAllocator = typename allocator_traits::template rebind_alloc<I>;
 
   size_type m_ = 0, n_ = 0, nnz_ = 0;
-  std::vector<T> vals_;
-  std::vector<I> row_ptr_;
-  std::vector<I> col_ind_;
+  std::vector<T, Allocator> vals_;
+  std::vector<I, IAllocator> row_ptr_;
+  std::vector<I, IAllocator> col_ind_;
 
   CSRMatrix() = default;
   CSRMatrix(size_type m, size_type n) : m_(m), n_(n), nnz_(0), row_ptr_(m + 1, 0) {}
   CSRMatrix(size_type m, size_type n, size_type nnz) : m_(m), n_(n), nnz_(nnz), vals_(nnz), row_ptr_(m + 1), col_ind_(nnz) {}
-  CSRMatrix(size_type m, size_type n, size_type nnz, std::vector<T>&& vals, std::vector<I>&& row_ptr,
-            std::vector<I>&& col_ind)
+  CSRMatrix(size_type m, size_type n, size_type nnz, std::vector<T, Allocator>&& vals,
+            std::vector<I, IAllocator>&& row_ptr, std::vector<I, IAllocator>&& col_ind)
       : m_(m), n_(n), nnz_(nnz), vals_(std::move(vals)), row_ptr_(std::move(row_ptr)), col_ind_(std::move(col_ind)) {}
-  explicit CSRMatrix(const std::string& fname) { read_Binary(fname); }
+  // From a file: ".mtx" is MatrixMarket, ".binary" BCL's binary format, unless the format is given
+  // (CSRMatrix.hpp:212-224, matrix_io::detect_file_type).
+  explicit CSRMatrix(const std::string& fname, FileFormat format = Unknown);
 
   size_type m() const { return m_; }
   size_type n() const { return n_; }
@@ -178,6 +200,11 @@ struct CSRMatrix {
     return r;
   }
 
+  // MatrixMarket coordinate file (1-based unless one_indexed is false), through libbclx's linear-time reader.
+  void read_MatrixMarket(const std::string& fname, bool one_indexed = true);
+  // The reference's output format: "integer general" banner, 1-based indices, values printed with "%f".
+  void write_MatrixMarket(const std::string& fname) const;
+
   // Binary CSR file: m, n, nnz as size_t, then vals (T), col_ind (I), row_ptr (I).
   void read_Binary(const std::string& fname) {
     FILE* f = fopen(fname.c_str(), "rb");
@@ -214,7 +241,8 @@ struct DevCSR {
   Dev<int32_t> col;
   Dev<T> val;
   bclx_csr d{};
-  explicit DevCSR(const CSRMatrix<T, I>& a)
+  template <class M>  // any container with BCL::CSRMatrix's public members
+  explicit DevCSR(const M& a)
       : rp(a.row_ptr_.data(), a.m_ + 1), col(a.nnz_), val(a.vals_.data(), a.nnz_) {
     std::vector<int32_t> c(a.col_ind_.begin(), a.col_ind_.end());
     if (a.nnz_) hip_check(hipMemcpy(col.p, c.data(), 4 * a.nnz_, hipMemcpyHostToDevice), "hipMemcpy H2D");
@@ -222,20 +250,25 @@ struct DevCSR {
   }
 };
 
-template <typename T, typename I>
-CSRMatrix<T, I> to_host(const Dev<I>& rp, const Dev<int32_t>& col, const Dev<T>& val, size_t m, size_t n,
-                        size_t nnz) {
-  CSRMatrix<T, I> c(m, n, nnz);
-  rp.to_host(c.row_ptr_.data(), m + 1);
+// A host matrix of type M (BCL::CSRMatrix's public members and its (m, n, nnz, vals, row_ptr, col_ind) constructor)
+// from device arrays.
+template <class M, typename T, typename I>
+M to_host(const Dev<I>& rp, const Dev<int32_t>& col, const Dev<T>& val, size_t m, size_t n, size_t nnz) {
+  decltype(M::vals_) v(nnz);
+  decltype(M::row_ptr_) r(m + 1);
+  decltype(M::col_ind_) c(nnz);
+  rp.to_host(r.data(), m + 1);
   std::vector<int32_t> cc(nnz);
   col.to_host(cc.data(), nnz);
-  std::copy(cc.begin(), cc.end(), c.col_ind_.begin());
-  val.to_host(c.vals_.data(), nnz);
-  return c;
+  std::copy(cc.begin(), cc.end(), c.begin());
+  val.to_host(v.data(), nnz);
+  return M(m, n, nnz, std::move(v), std::move(r), std::move(c));
 }
 
-template <typename T, typename I>
-CSRMatrix<T, I> multiply(const CSRMatrix<T, I>& a, const CSRMatrix<T, I>& b) {
+template <class M>
+M multiply(const M& a, const M& b) {
+  using T = typename M::value_type;
+  using I = typename M::index_type;
   if (a.n_ != b.m_) throw std::runtime_error("bclx: inner dimensions differ");
   bclx_handle* h = ctx().handle();
   DevCSR<T, I> A(a), B(b);
@@ -249,11 +282,13 @@ CSRMatrix<T, I> multiply(const CSRMatrix<T, I>& a, const CSRMatrix<T, I>& b) {
   bclx_csr C{(int64_t)a.m_, (int64_t)b.n_, nnz, width_of<I>(), dtype_of<T>(), rp.p, col.p, val.p};
   check(bclx_spgemm_numeric(plan, &C));
   check(bclx_synchronize(h));
-  return to_host<T, I>(rp, col, val, a.m_, b.n_, (size_t)nnz);
+  return to_host<M>(rp, col, val, a.m_, b.n_, (size_t)nnz);
 }
 
-template <typename T, typename I>
-CSRMatrix<T, I> sum(const std::vector<const CSRMatrix<T, I>*>& mats, size_t m, size_t n) {
+template <class M>
+M sum(const std::vector<const M*>& mats, size_t m, size_t n) {
+  using T = typename M::value_type;
+  using I = typename M::index_type;
   bclx_handle* h = ctx().handle();
   std::vector<std::unique_ptr<DevCSR<T, I>>> dev;
   std::vector<bclx_csr> descs;
@@ -274,39 +309,259 @@ CSRMatrix<T, I> sum(const std::vector<const CSRMatrix<T, I>*>& mats, size_t m, s
     check(bclx_accumulate_tiles(h, (int)descs.size(), descs.data(), &C));
   }
   check(bclx_synchronize(h));
-  return to_host<T, I>(rp, col, val, m, n, (size_t)nnz);
+  return to_host<M>(rp, col, val, m, n, (size_t)nnz);
 }
+
+// Containers shaped like BCL::CSRMatrix (public vals_ / row_ptr_ / col_ind_ and value_type / index_type).
+template <class M, class = void>
+struct is_csr : std::false_type {};
+template <class M>
+struct is_csr<M, std::void_t<typename M::value_type, typename M::index_type, decltype(std::declval<M&>().vals_),
+                             decltype(std::declval<M&>().row_ptr_), decltype(std::declval<M&>().col_ind_)>>
+    : std::true_type {};
 
 }  // namespace detail
 
-template <typename T, typename I>
-CSRMatrix<T, I> CSRMatrix<T, I>::dot(const CSRMatrix& other) const {
+template <typename T, typename I, typename A>
+CSRMatrix<T, I, A> CSRMatrix<T, I, A>::dot(const CSRMatrix& other) const {
   return detail::multiply(*this, other);
 }
 
-// c += a * b (BCL::gemm(CSRMatrix, CSRMatrix, CSRMatrix)): the existing entries of c are kept and summed with the
-// product; new entries are created for every product position, including exact-zero sums.
+// a * b on the GPU for any matrix type shaped like BCL::CSRMatrix<T, I, Allocator>, returned as that same type: the
+// call `my_a.dot(my_b)` of BCL::gemm (spgemm.hpp:221) becomes `bclx::dot(my_a, my_b)` on BCL's own tiles, with no
+// conversion of the operands or the result.
+template <class M, typename = std::enable_if_t<detail::is_csr<M>::value>>
+M dot(const M& a, const M& b) {
+  return detail::multiply(a, b);
+}
+
+namespace matrix_io {
+
+inline FileFormat detect_file_type(const std::string& fname) {
+  const size_t dot = fname.rfind('.');
+  const std::string suffix = dot == std::string::npos ? std::string() : fname.substr(dot);
+  if (suffix == ".mtx") return MatrixMarket;
+  if (suffix == ".binary") return Binary;
+  return Unknown;
+}
+
+struct matrix_shape {
+  std::array<size_t, 2> shape;
+  size_t nnz;
+};
+
+// m, n and the stored entry count of a MatrixMarket or binary file (matrix_io::matrix_info, matrix_io.hpp:50-113)
+inline matrix_shape matrix_info(const std::string& fname, FileFormat format = Unknown) {
+  if (format == Unknown) format = detect_file_type(fname);
+  if (format == MatrixMarket || format == MatrixMarketZeroIndexed) {
+    int64_t m = 0, n = 0, nnz = 0;
+    detail::check(bclx_mm_info(fname.c_str(), format == MatrixMarket, &m, &n, &nnz));
+    return {{(size_t)m, (size_t)n}, (size_t)nnz};
+  }
+  if (format == Binary) {
+    FILE* f = fopen(fname.c_str(), "rb");
+    if (!f) throw std::runtime_error("bclx: cannot open " + fname);
+    size_t h[3] = {0, 0, 0};
+    const bool ok = fread(h, sizeof(size_t), 3, f) == 3;
+    fclose(f);
+    if (!ok) throw std::runtime_error("bclx: short read in " + fname);
+    return {{h[0], h[1]}, h[2]};
+  }
+  throw std::runtime_error("matrix_info: Could not detect file format for \"" + fname + "\"");
+}
+
+}  // namespace matrix_io
+
+template <typename T, typename I, typename A>
+CSRMatrix<T, I, A>::CSRMatrix(const std::string& fname, FileFormat format) {
+  if (format == Unknown) format = matrix_io::detect_file_type(fname);
+  if (format == MatrixMarket) read_MatrixMarket(fname);
+  else if (format == MatrixMarketZeroIndexed) read_MatrixMarket(fname, false);
+  else if (format == Binary) read_Binary(fname);
+  else throw std::runtime_error("CSRMatrix: Could not detect file format for \"" + fname + "\"");
+}
+
+template <typename T, typename I, typename A>
+void CSRMatrix<T, I, A>::read_MatrixMarket(const std::string& fname, bool one_indexed) {
+  bclx_csr c{};
+  detail::check(bclx_mm_read(fname.c_str(), one_indexed, detail::dtype_of<T>(), 8, &c));
+  std::unique_ptr<void, void (*)(void*)> rp(c.row_ptr, bclx_host_free), col(c.col_idx, bclx_host_free),
+      val(c.vals, bclx_host_free);
+  m_ = (size_type)c.m;
+  n_ = (size_type)c.n;
+  nnz_ = (size_type)c.nnz;
+  const int64_t* r = static_cast<const int64_t*>(c.row_ptr);
+  row_ptr_.assign(r, r + m_ + 1);
+  col_ind_.assign(c.col_idx, c.col_idx + nnz_);
+  const T* v = static_cast<const T*>(c.vals);
+  vals_.assign(v, v + nnz_);
+}
+
+template <typename T, typename I, typename A>
+void CSRMatrix<T, I, A>::write_MatrixMarket(const std::string& fname) const {
+  std::vector<int64_t> rp(row_ptr_.begin(), row_ptr_.end());
+  std::vector<int32_t> col(col_ind_.begin(), col_ind_.end());
+  bclx_csr c{(int64_t)m_, (int64_t)n_, (int64_t)nnz_, 8, detail::dtype_of<T>(), rp.data(), col.data(),
+             const_cast<T*>(vals_.data())};
+  detail::check(bclx_mm_write(fname.c_str(), &c));
+}
+
+// BCL's binary CSR file mapped read-only into memory (CSRMatrixMemoryMapped, CSRMatrixMemoryMapped.hpp:55-145): the
+// arrays are views of the file, paged in as they are used; get_slice_impl_ copies one tile out.
 template <typename T, typename I>
-void gemm(const CSRMatrix<T, I>& a, const CSRMatrix<T, I>& b, CSRMatrix<T, I>& c) {
+class CSRMatrixMemoryMapped {
+ public:
+  using value_type = T;
+  using index_type = I;
+  using size_type = size_t;
+
+  explicit CSRMatrixMemoryMapped(const std::string& fname) {
+    fd_ = open(fname.c_str(), O_RDONLY);
+    if (fd_ < 0) throw std::runtime_error("CSRMatrixMemoryMapped: could not open file \"" + fname + "\"");
+    struct stat st;
+    if (fstat(fd_, &st) != 0 || (size_t)st.st_size < 3 * sizeof(size_t)) {
+      close(fd_);
+      throw std::runtime_error("CSRMatrixMemoryMapped: \"" + fname + "\" is not a binary CSR file");
+    }
+    length_ = (size_t)st.st_size;
+    void* a = mmap(nullptr, length_, PROT_READ, MAP_SHARED, fd_, 0);
+    if (a == MAP_FAILED) {
+      close(fd_);
+      throw std::runtime_error("CSRMatrixMemoryMapped: could not map \"" + fname + "\"");
+    }
+    address_ = a;
+    const size_t* h = static_cast<const size_t*>(address_);
+    m_ = h[0];
+    n_ = h[1];
+    nnz_ = h[2];
+    if (3 * sizeof(size_t) + nnz_ * (sizeof(T) + sizeof(I)) + (m_ + 1) * sizeof(I) > length_) {
+      munmap(address_, length_);
+      close(fd_);
+      throw std::runtime_error("CSRMatrixMemoryMapped: \"" + fname + "\" is shorter than its header says");
+    }
+    values_ = reinterpret_cast<const T*>(h + 3);
+    colind_ = reinterpret_cast<const I*>(values_ + nnz_);
+    rowptr_ = colind_ + nnz_;
+  }
+  CSRMatrixMemoryMapped(const CSRMatrixMemoryMapped&) = delete;
+  CSRMatrixMemoryMapped& operator=(const CSRMatrixMemoryMapped&) = delete;
+  ~CSRMatrixMemoryMapped() {
+    munmap(address_, length_);
+    close(fd_);
+  }
+
+  const T* values_data() const { return values_; }
+  const I* colind_data() const { return colind_; }
+  const I* rowptr_data() const { return rowptr_; }
+  size_t m() const { return m_; }
+  size_t n() const { return n_; }
+  size_t nnz() const { return nnz_; }
+
+  CSRMatrix<T, I> get_slice_impl_(size_type imin, size_type imax, size_type jmin, size_type jmax) const {
+    imax = std::min(imax, m_);
+    jmax = std::min(jmax, n_);
+    CSRMatrix<T, I> r(imax - imin, jmax - jmin);
+    for (size_type i = imin; i < imax; i++) {
+      for (I p = rowptr_[i]; p < rowptr_[i + 1]; p++) {
+        const size_type j = (size_type)colind_[p];
+        if (j >= jmin && j < jmax) {
+          r.vals_.push_back(values_[p]);
+          r.col_ind_.push_back((I)(j - jmin));
+        }
+      }
+      r.row_ptr_[i - imin + 1] = (I)r.vals_.size();
+    }
+    r.nnz_ = r.vals_.size();
+    return r;
+  }
+
+ private:
+  int fd_ = -1;
+  void* address_ = nullptr;
+  size_t length_ = 0;
+  size_t m_ = 0, n_ = 0, nnz_ = 0;
+  const T* values_ = nullptr;
+  const I* colind_ = nullptr;
+  const I* rowptr_ = nullptr;
+};
+
+// The tiles of a on a grid of tile_m x tile_n tiles, cut on the GPU (row-major over the grid): the slicing of
+// SPMatrix's constructor (SPMatrix.hpp:212-266).
+template <class M, typename = std::enable_if_t<detail::is_csr<M>::value>>
+std::vector<M> split_grid(const M& a, size_t tile_m, size_t tile_n) {
+  using T = typename M::value_type;
+  using I = typename M::index_type;
+  bclx_handle* h = detail::ctx().handle();
+  detail::DevCSR<T, I> A(a);
+  const size_t gm = std::max<size_t>(1, (a.m_ + tile_m - 1) / tile_m), gn = std::max<size_t>(1, (a.n_ + tile_n - 1) / tile_n);
+  std::vector<bclx_csr> d(gm * gn);
+  for (auto& x : d) x = bclx_csr{};
+  detail::check(bclx_csr_split_grid(h, &A.d, (int64_t)tile_m, (int64_t)tile_n, (int32_t)d.size(), d.data()));
+  std::vector<std::unique_ptr<detail::Dev<I>>> rp;
+  std::vector<std::unique_ptr<detail::Dev<int32_t>>> col;
+  std::vector<std::unique_ptr<detail::Dev<T>>> val;
+  for (auto& x : d) {
+    rp.push_back(std::make_unique<detail::Dev<I>>((size_t)x.m + 1));
+    col.push_back(std::make_unique<detail::Dev<int32_t>>((size_t)x.nnz));
+    val.push_back(std::make_unique<detail::Dev<T>>((size_t)x.nnz));
+    x.ptr_width = detail::width_of<I>();
+    x.row_ptr = rp.back()->p;
+    x.col_idx = col.back()->p;
+    x.vals = val.back()->p;
+  }
+  detail::check(bclx_csr_split_grid(h, &A.d, (int64_t)tile_m, (int64_t)tile_n, (int32_t)d.size(), d.data()));
+  detail::check(bclx_synchronize(h));
+  std::vector<M> out;
+  for (size_t t = 0; t < d.size(); t++)
+    out.push_back(detail::to_host<M>(*rp[t], *col[t], *val[t], (size_t)d[t].m, (size_t)d[t].n, (size_t)d[t].nnz));
+  return out;
+}
+
+// Elementwise tile arithmetic of DMatrix (DMatrix.hpp:319-386) on host tiles, run on the GPU: c = a op b for
+// op in {'+', '-', '*', '/'}; c may be a (binary_op_inplace, operator+=).
+template <typename T>
+void tile_binary_op(char op, size_t m, size_t n, const T* a, size_t lda, const T* b, size_t ldb, T* c, size_t ldc) {
+  const int code = op == '+' ? 0 : op == '-' ? 1 : op == '*' ? 2 : op == '/' ? 3 : -1;
+  if (code < 0) throw std::runtime_error("bclx: tile_binary_op takes '+', '-', '*' or '/'");
+  bclx_handle* h = detail::ctx().handle();
+  detail::Dev<T> A(a, m * lda), B(b, m * ldb), C(m * ldc);
+  if (c != a && ldc > n) detail::hip_check(hipMemcpy(C.p, c, sizeof(T) * m * ldc, hipMemcpyHostToDevice), "hipMemcpy");
+  detail::check(bclx_dense_binary_op(h, code, detail::dtype_of<T>(), (int64_t)m, (int64_t)n, A.p, (int64_t)lda, B.p,
+                                     (int64_t)ldb, c == a ? A.p : C.p, (int64_t)(c == a ? lda : ldc)));
+  detail::check(bclx_synchronize(h));
+  (c == a ? A : C).to_host(c, m * (c == a ? lda : ldc));
+}
+
+// c += a * b (BCL::gemm(CSRMatrix, CSRMatrix, CSRMatrix)): the existing entries of c are kept and summed with the
+// product; new entries are created for every product position, including exact-zero sums. M is bclx::CSRMatrix or
+// BCL::CSRMatrix<T, I, Allocator> itself.
+template <class M, typename = std::enable_if_t<detail::is_csr<M>::value>>
+void gemm(const M& a, const M& b, M& c) {
   if (c.m_ != a.m_ || c.n_ != b.n_) throw std::runtime_error("bclx: c has the wrong shape");
-  CSRMatrix<T, I> p = detail::multiply(a, b);
+  M p = detail::multiply(a, b);
   if (c.nnz_ == 0) {
     c = std::move(p);
     return;
   }
-  c = detail::sum<T, I>({&c, &p}, c.m_, c.n_);
+  c = detail::sum<M>({&c, &p}, c.m_, c.n_);
 }
 
-// BCL::SparseHashAccumulator: collect partial C tiles, then get_matrix(m, n) sums them (on the GPU).
-template <typename T, typename I = int>
+// BCL::SparseHashAccumulator<T, I, Allocator> (SparseHashAccumulator.hpp:86-183): collect partial C tiles, then
+// get_matrix(m, n) sums them on the GPU. M is the tile type (BCL::CSRMatrix<T, I, Allocator> works as well).
+template <typename T, typename I = int, typename Allocator = std::allocator<T>, class M = CSRMatrix<T, I, Allocator>>
 struct SparseHashAccumulator {
-  std::vector<CSRMatrix<T, I>> mats_;
-  void accumulate(CSRMatrix<T, I>&& mat) { mats_.push_back(std::move(mat)); }
-  CSRMatrix<T, I> get_matrix(size_t m, size_t n) const {
-    if (mats_.empty()) return CSRMatrix<T, I>(m, n);
-    std::vector<const CSRMatrix<T, I>*> p;
+  std::vector<M> mats_;
+  void accumulate(M&& mat) { mats_.push_back(std::move(mat)); }
+  M get_matrix(size_t m, size_t n) const {
+    if (mats_.empty()) {
+      decltype(M::vals_) v;
+      decltype(M::row_ptr_) r(m + 1, 0);
+      decltype(M::col_ind_) c;
+      return M(m, n, 0, std::move(v), std::move(r), std::move(c));
+    }
+    std::vector<const M*> p;
     for (auto& x : mats_) p.push_back(&x);
-    return detail::sum<T, I>(p, m, n);
+    return detail::sum<M>(p, m, n);
   }
 };
 

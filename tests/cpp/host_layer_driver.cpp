@@ -1,15 +1,60 @@
 // Drives the C++ host layer (include/bclx/bclx.hpp) the way BCL code calls the local tile multiply, on files
 // written by tests/test_cpp_layer_gpu.py in BCL's binary CSR format:
-//   <dir>/a.bin, b.bin, c0.bin  ->  dot.bin = a.dot(b);  gemm.bin = c0 after bclx::gemm(a, b, c0)  (c0 += a * b)
+//   <dir>/a.binary, b.binary, c0.binary -> dot.bin = a.dot(b) (also dot_alloc.bin with a custom allocator, and
+//                                   dot_like.bin / gemm_like.bin / acc_like.bin on a type with only BCL's members);  gemm.bin = c0 after bclx::gemm(a, b, c0)  (c0 += a * b)
 //                                   acc.bin = SparseHashAccumulator over {a*b, c0}
 //   <dir>/dense.bin (m, n, k, then A m x k, B k x n, C m x n as float) -> gemm_out.bin = C + A * B (cblas wrapper,
 //   row-major), gemm_cm.bin = the same product through the column-major path
 //   spmm: spmm_in.bin (n, ld, then B k x ld, C m x ld) with a -> spmm_out.bin (C += a * B[:, :n])
+//   files: a.mtx -> mtx.binary (CSRMatrix(fname) of a MatrixMarket file), mtx_out.mtx (write_MatrixMarket);
+//   a.binary memory-mapped -> slice.binary = rows [100, 700) x columns [50, 1500); split_grid(a, 300, 500) ->
+//   split_<t>.binary; ew.bin (m, n, ld, then X, Y m x ld) -> ew_out.bin = X + Y, X - Y, X * Y, X / Y, then X += Y
 #include <bclx/bclx.hpp>
 
 #include <cstdio>
 #include <string>
 #include <vector>
+
+// Containers other than bclx::CSRMatrix<float, int>: one with a custom allocator (the reference's Allocator template
+// parameter), and a type that only has BCL::CSRMatrix's public members and constructor, as BCL's own tiles do.
+template <class T>
+struct TrackingAlloc {
+  using value_type = T;
+  static inline size_t live = 0;
+  TrackingAlloc() = default;
+  template <class U>
+  TrackingAlloc(const TrackingAlloc<U>&) {}
+  T* allocate(size_t n) {
+    live += n;
+    return std::allocator<T>().allocate(n);
+  }
+  void deallocate(T* p, size_t n) {
+    live -= n;
+    std::allocator<T>().deallocate(p, n);
+  }
+  template <class U>
+  bool operator==(const TrackingAlloc<U>&) const { return true; }
+  template <class U>
+  bool operator!=(const TrackingAlloc<U>&) const { return false; }
+};
+
+struct LikeBCL {
+  using value_type = float;
+  using index_type = int;
+  size_t m_, n_, nnz_;
+  std::vector<float> vals_;
+  std::vector<int> row_ptr_, col_ind_;
+  LikeBCL(size_t m, size_t n, size_t nnz, std::vector<float>&& v, std::vector<int>&& r, std::vector<int>&& c)
+      : m_(m), n_(n), nnz_(nnz), vals_(std::move(v)), row_ptr_(std::move(r)), col_ind_(std::move(c)) {}
+};
+
+template <class M, class S>
+static M convert(const S& x) {
+  decltype(M::vals_) v(x.vals_.begin(), x.vals_.end());
+  decltype(M::row_ptr_) r(x.row_ptr_.begin(), x.row_ptr_.end());
+  decltype(M::col_ind_) c(x.col_ind_.begin(), x.col_ind_.end());
+  return M(x.m_, x.n_, x.nnz_, std::move(v), std::move(r), std::move(c));
+}
 
 template <typename T>
 static std::vector<T> read_vec(FILE* f, size_t n) {
@@ -25,7 +70,7 @@ int main(int argc, char** argv) {
   }
   const std::string d = argv[1];
   try {
-    bclx::CSRMatrix<float, int> a(d + "/a.bin"), b(d + "/b.bin"), c0(d + "/c0.bin");
+    bclx::CSRMatrix<float, int> a(d + "/a.binary"), b(d + "/b.binary"), c0(d + "/c0.binary");
     a.dot(b).write_Binary(d + "/dot.bin");
     bclx::CSRMatrix<float, int> c = c0;
     bclx::gemm(a, b, c);
@@ -34,6 +79,20 @@ int main(int argc, char** argv) {
     acc.accumulate(a.dot(b));
     acc.accumulate(bclx::CSRMatrix<float, int>(c0));
     acc.get_matrix(a.m(), b.n()).write_Binary(d + "/acc.bin");
+    {  // the same products on other container types
+      using AllocM = bclx::CSRMatrix<float, int, TrackingAlloc<float>>;
+      AllocM ta = convert<AllocM>(a), tb = convert<AllocM>(b);
+      convert<bclx::CSRMatrix<float, int>>(ta.dot(tb)).write_Binary(d + "/dot_alloc.bin");
+      if (TrackingAlloc<float>::live == 0) throw std::runtime_error("the custom allocator was not used");
+      LikeBCL la = convert<LikeBCL>(a), lb = convert<LikeBCL>(b), lc = convert<LikeBCL>(c0);
+      convert<bclx::CSRMatrix<float, int>>(bclx::dot(la, lb)).write_Binary(d + "/dot_like.bin");
+      bclx::gemm(la, lb, lc);
+      convert<bclx::CSRMatrix<float, int>>(lc).write_Binary(d + "/gemm_like.bin");
+      bclx::SparseHashAccumulator<float, int, std::allocator<float>, LikeBCL> lacc;
+      lacc.accumulate(bclx::dot(la, lb));
+      lacc.accumulate(convert<LikeBCL>(c0));
+      convert<bclx::CSRMatrix<float, int>>(lacc.get_matrix(a.m(), b.n())).write_Binary(d + "/acc_like.bin");
+    }
 
     FILE* f = fopen((d + "/dense.bin").c_str(), "rb");
     auto dims = read_vec<int64_t>(f, 3);
@@ -71,6 +130,31 @@ int main(int argc, char** argv) {
                             a.col_ind_.data(), ld, ld);
     o = fopen((d + "/spmm_out.bin").c_str(), "wb");
     fwrite(SC.data(), sizeof(float), SC.size(), o);
+    fclose(o);
+
+    bclx::CSRMatrix<float, int> mtx(d + "/a.mtx");
+    mtx.write_Binary(d + "/mtx.binary");
+    mtx.write_MatrixMarket(d + "/mtx_out.mtx");
+    auto info = bclx::matrix_io::matrix_info(d + "/a.mtx");
+    if (info.shape[0] != mtx.m() || info.shape[1] != mtx.n()) throw std::runtime_error("matrix_info disagrees");
+    bclx::CSRMatrixMemoryMapped<float, int> mm(d + "/a.binary");
+    mm.get_slice_impl_(100, 700, 50, 1500).write_Binary(d + "/slice.binary");
+    auto tiles = bclx::split_grid(a, 300, 500);
+    for (size_t t = 0; t < tiles.size(); t++) tiles[t].write_Binary(d + "/split_" + std::to_string(t) + ".binary");
+
+    f = fopen((d + "/ew.bin").c_str(), "rb");
+    auto ed = read_vec<int64_t>(f, 3);
+    const size_t em = (size_t)ed[0], en = (size_t)ed[1], eld = (size_t)ed[2];
+    auto X = read_vec<double>(f, em * eld), Y = read_vec<double>(f, em * eld);
+    fclose(f);
+    o = fopen((d + "/ew_out.bin").c_str(), "wb");
+    for (char op : std::string("+-*/")) {
+      std::vector<double> Z(em * eld, -1.0);
+      bclx::tile_binary_op<double>(op, em, en, X.data(), eld, Y.data(), eld, Z.data(), eld);
+      fwrite(Z.data(), sizeof(double), Z.size(), o);
+    }
+    bclx::tile_binary_op<double>('+', em, en, X.data(), eld, Y.data(), eld, X.data(), eld);  // X += Y
+    fwrite(X.data(), sizeof(double), X.size(), o);
     fclose(o);
   } catch (const std::exception& e) {
     fprintf(stderr, "host_layer_driver: %s\n", e.what());

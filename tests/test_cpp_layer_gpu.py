@@ -1,6 +1,8 @@
 """The C++ host layer (include/bclx/bclx.hpp) driven like BCL code: CSRMatrix::dot, gemm (c += a * b),
 SparseHashAccumulator, spmm_wrapper and cblas_gemm_wrapper_ (row- and column-major), through files in BCL's binary
-CSR format. Results are checked against the oracle with the same bounds as the C-ABI tests."""
+CSR format; CSRMatrix(fname) of a MatrixMarket file and write_MatrixMarket, CSRMatrixMemoryMapped::get_slice_impl_,
+split_grid and the DMatrix tile arithmetic. Results are checked against the oracle with the same bounds as the C-ABI
+tests, and against the reference's own readers and slicers where oracle/_ref is built."""
 import os
 import subprocess
 
@@ -9,7 +11,7 @@ import pytest
 
 import oracle_py as O
 from bcl_amd import gen
-from bcl_amd.csr import CSR, read_binary, write_binary
+from bcl_amd.csr import CSR, get_slice, read_binary, read_matrix_market, write_binary, write_matrix_market
 
 pytestmark = pytest.mark.gpu
 DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "host_layer_driver")
@@ -26,7 +28,14 @@ def test_cpp_host_layer(tmp_path):
     b = gen.uniform(a.n, 900, 6, seed=22)
     c0 = gen.uniform(a.m, 900, 3, seed=23)
     for name, x in (("a", a), ("b", b), ("c0", c0)):
-        write_binary(x, str(tmp_path / f"{name}.bin"))
+        write_binary(x, str(tmp_path / f"{name}.binary"))
+    write_matrix_market(a, str(tmp_path / "a.mtx"))
+    em, en, eld = 37, 50, 56
+    X = np.random.default_rng(5).standard_normal((em, eld))
+    Y = np.random.default_rng(6).standard_normal((em, eld)) + 3.0
+    with open(tmp_path / "ew.bin", "wb") as f:
+        np.array([em, en, eld], np.int64).tofile(f)
+        X.tofile(f), Y.tofile(f)
     rng = np.random.default_rng(0)
     m, n, k = 130, 70, 90
     A = rng.standard_normal((m, k)).astype(np.float32)
@@ -53,6 +62,13 @@ def test_cpp_host_layer(tmp_path):
     u = 2.0 ** -24
     check_sparse(read_binary(str(tmp_path / "dot.bin")), O.spgemm(a, b), O.spgemm_f64(a, b),
                  O.spgemm_f64(_abs(a), _abs(b)), u)
+    dot = read_binary(str(tmp_path / "dot.bin"))
+    for other, same_as in (("dot_alloc.bin", "dot.bin"), ("dot_like.bin", "dot.bin"), ("gemm_like.bin", "gemm.bin"),
+                           ("acc_like.bin", "acc.bin")):
+        x, y = read_binary(str(tmp_path / other)), read_binary(str(tmp_path / same_as))
+        np.testing.assert_array_equal(x.row_ptr, y.row_ptr)
+        np.testing.assert_array_equal(x.col_idx, y.col_idx)
+        np.testing.assert_array_equal(x.vals, y.vals)
     ref = O.spgemm(a, b, c0)
     ref64 = O.spgemm_f64(a, b)
     # c0 + a*b on the reference's structure, values within two roundings of the float64 sum
@@ -74,3 +90,27 @@ def test_cpp_host_layer(tmp_path):
     mag = np.abs(SC[:, :sn]) + O.spmm_f64(_abs(a), np.ascontiguousarray(np.abs(SB[:, :sn])))
     assert np.all(np.abs(got[:, :sn] - exp) <= 1e-5 * mag + 1e-30)
     np.testing.assert_array_equal(got[:, sn:], SC[:, sn:])
+
+    # files and tiles
+    mtx = read_binary(str(tmp_path / "mtx.binary"))
+    want = read_matrix_market(str(tmp_path / "a.mtx"))
+    for x, y in ((mtx, want), (read_binary(str(tmp_path / "slice.binary")), get_slice(a, 100, 700, 50, 1500))):
+        np.testing.assert_array_equal(x.row_ptr, y.row_ptr)
+        np.testing.assert_array_equal(x.col_idx, y.col_idx)
+        np.testing.assert_array_equal(x.vals, y.vals)
+    assert open(tmp_path / "mtx_out.mtx", "rb").read() == open(tmp_path / "a.mtx", "rb").read()
+    if O.have_ref():
+        r = O.ref_mmap_slice(str(tmp_path / "a.binary"), 100, 700, 50, 1500)
+        np.testing.assert_array_equal(read_binary(str(tmp_path / "slice.binary")).col_idx, r.col_idx)
+    gm, gn = -(-a.m // 300), -(-a.n // 500)
+    for t in range(gm * gn):
+        i, j = divmod(t, gn)
+        x, y = read_binary(str(tmp_path / f"split_{t}.binary")), get_slice(a, i * 300, (i + 1) * 300, j * 500,
+                                                                           (j + 1) * 500)
+        np.testing.assert_array_equal(x.row_ptr, y.row_ptr)
+        np.testing.assert_array_equal(x.col_idx, y.col_idx)
+        np.testing.assert_array_equal(x.vals, y.vals)
+    ew = np.fromfile(tmp_path / "ew_out.bin", np.float64).reshape(5, em, eld)
+    for q, want_ew in enumerate((X + Y, X - Y, X * Y, X / Y, X + Y)):
+        np.testing.assert_array_equal(ew[q][:, :en], want_ew[:, :en])
+    assert (ew[0][:, en:] == -1.0).all() and (ew[4][:, en:] == X[:, en:]).all()
