@@ -189,8 +189,8 @@ void split_grid(bclx_handle* h, const bclx_csr* A, int64_t tile_m, int64_t tile_
                        static_cast<const T*>(A->vals), m, A->n, tile_m, tile_n, gn, off.as<int64_t>(), ostride,
                        dt.as<SplitTile>(), bad);
     BCLX_HIP(hipGetLastError());
-  } else {  // one tile of no rows: its row_ptr is {0}
-    BCLX_HIP(hipMemsetAsync(tiles[0].row_ptr, 0, (size_t)tiles[0].ptr_width, s));
+  } else {  // one row of tiles with no rows: each row_ptr is {0}
+    for (int t = 0; t < gm * gn; t++) BCLX_HIP(hipMemsetAsync(tiles[t].row_ptr, 0, (size_t)tiles[t].ptr_width, s));
   }
   int32_t h_bad = 0;
   BCLX_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));

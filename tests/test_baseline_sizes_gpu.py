@@ -2,11 +2,12 @@
 
 Where the oracle cannot redo a whole result in seconds, the test compares every row's entry count with the oracle's
 (an exact count of the pattern of C), and the full structure and values of sampled rows: the heaviest rows of each
-product, rows whose outputs reach the last 64 columns of the tile, and random rows. Properties that do not depend on
+product, rows whose outputs reach within 64 of the last column B holds, and random rows. Properties that do not depend on
 size cover the rest (a closed form for the Laplacian's square; FP64 results for the dense product).
 
 - One SUMMA stage (k = 3, and the nine-times heavier k = 2) of R-MAT scale 22 (edge factor 16) on an 8 x 8 grid of
-  524288^2 tiles (the full 2^19-column width of the sort keys and the heavy-row bitmap), batched learned pipeline with
+  524288^2 tiles (2^19 columns: the width of the sort keys and the heavy-row bitmap, whose last columns the thin
+  products of tests/test_wide_tiles_gpu.py fill), batched learned pipeline with
   the trained predictor and the autotuned knobs, one graph launch (bcl/containers/algorithms/spgemm.hpp:181-242).
 - SpMM: R-MAT scale 20 times a dense 2^20 x 128 B, fp32 and bf16 B (spgemm.hpp:78-96).
 - Dense GEMM 8192^3 in the three MFMA modes against the FP64 product (cblas_wrapper.hpp:16-40).
@@ -54,14 +55,15 @@ def row_products(a: CSR, b: CSR) -> np.ndarray:
 
 
 def sample_rows(a: CSR, b: CSR, n_heavy: int, n_edge: int, n_rand: int, seed: int) -> np.ndarray:
-    """The n_heavy rows with the most products, n_edge random rows whose output reaches the last 64 columns, and
-    n_rand random non-empty rows."""
+    """The n_heavy rows with the most products, n_edge random rows whose output reaches within 64 of the largest
+    column B holds (R-MAT leaves the last columns of a tile empty: at scale 22 the largest held column of a 2^19-wide
+    tile is about 2^19 - 250), and n_rand random non-empty rows."""
     rng = np.random.default_rng(seed)
     prod = row_products(a, b)
     heavy = np.argsort(prod, kind="stable")[-n_heavy:]
     blen = np.diff(b.row_ptr)
     last = np.where(blen > 0, b.col_idx[np.maximum(b.row_ptr[1:] - 1, 0)], -1)
-    reach = (last >= b.n - 64).astype(np.int64)
+    reach = (last >= int(last.max()) - 64).astype(np.int64)
     cum = np.concatenate([[0], np.cumsum(reach[a.col_idx])])
     edge_rows = np.nonzero(cum[a.row_ptr[1:]] - cum[a.row_ptr[:-1]] > 0)[0]
     edge = rng.choice(edge_rows, min(n_edge, edge_rows.size), replace=False) if edge_rows.size else edge_rows
@@ -107,9 +109,12 @@ def check_product_sampled(plan, p, a, b, n_heavy, seed):
     err = np.abs(gval - ref.vals)
     bound = spgemm_value_bound(sub, b, ref.vals, absum.vals, np.float32)
     assert (err <= bound).all(), f"product {p}: {(err > bound).sum()} values out of bound"
+    b_last = int(b.col_idx.max()) if b.nnz else -1
+    if n_edge_rows:
+        assert gcol.max() >= b_last - 64, f"product {p}: the sampled rows must reach the last columns B holds"
     return {"rows": int(rows.size), "entries": int(gcol.size), "edge_rows": int(n_edge_rows),
             "max_rel": float((err / np.maximum(np.abs(ref.vals), 1e-300)).max()) if err.size else 0.0,
-            "max_col": int(gcol.max()) if gcol.size else -1}
+            "max_col": int(gcol.max()) if gcol.size else -1, "b_max_col": b_last}
 
 
 @pytest.fixture(scope="module")
@@ -132,13 +137,13 @@ def test_stage_rmat22_grid8(handle, rmat22_tiles, k, n_heavy):
     torch.cuda.synchronize()
     st = plan.stats()
     print(f"[s22] stats {st}", flush=True)
-    max_col = -1
+    edge = 0
     for p, (a, b) in enumerate(pairs):
         info = check_product_sampled(plan, p, a, b, n_heavy, seed=100 * k + p)
-        max_col = max(max_col, info["max_col"])
+        edge += info["edge_rows"] > 0
         if p % 8 == 0:
             print(f"[s22] k={k} product {p}: {info}", flush=True)
-    assert max_col >= (1 << 19) - 64, "the sampled rows must reach the last columns of the tiles"
+    assert edge >= len(pairs) // 2, "most products must have sampled rows at the right edge of their columns"
     assert st["heavy_rows"] > 0 and st["sorted_rows"] > 0
     plan.close()
     del A, B

@@ -70,8 +70,9 @@ def test_split_grid_empty_and_single(handle):
     t = split_grid(handle, DeviceCSR.from_host(e), 4, 8)
     assert len(t) == 3 * 3 and all(x.nnz == 0 for x in t.values())
     assert all(int(x.row_ptr.abs().sum()) == 0 and x.row_ptr.numel() == x.m + 1 for x in t.values())
-    z = split_grid(handle, DeviceCSR.from_host(CSR.empty(0, 5)), 4, 4)
-    assert list(z) == [(0, 0)] and z[(0, 0)].m == 0 and int(z[(0, 0)].row_ptr[0]) == 0
+    z = split_grid(handle, DeviceCSR.from_host(CSR.empty(0, 5)), 4, 4)  # no rows: one row of two tiles
+    assert sorted(z) == [(0, 0), (0, 1)] and [z[k].n for k in sorted(z)] == [4, 1]
+    assert all(x.m == 0 and x.row_ptr.numel() == 1 and int(x.row_ptr[0]) == 0 for x in z.values())
     a = gen.uniform(100, 80, 5, seed=2)
     one = split_grid(handle, DeviceCSR.from_host(a), 100, 80)
     assert_tile(one[(0, 0)].to_host(), a, "one tile")
