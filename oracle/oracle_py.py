@@ -197,8 +197,12 @@ def spmm(a: CSR, b: np.ndarray, c: np.ndarray) -> None:
 
 
 def spmm_f64(a: CSR, b: np.ndarray, c0: np.ndarray | None = None, nthreads: int = 0) -> np.ndarray:
-    """float64 c0 + a @ b; b is float32 or bf16 bits (uint16)."""
+    """float64 c0 + a @ b; b is float32 or bf16 bits (uint16); c0 (optional) is taken as float32."""
     arp, acol = _c64(a)
+    assert a.vals.dtype == np.float32 and b.dtype in (np.float32, np.uint16), "fp32 A and fp32 / bf16-bits B"
+    if c0 is not None:
+        c0 = np.ascontiguousarray(c0, np.float32)
+        assert c0.shape == (a.m, b.shape[1])
     n = b.shape[1]
     out = np.zeros((a.m, n), np.float64)
     bf = b if b.dtype == np.float32 else None
