@@ -214,8 +214,11 @@ def test_gemm_8192_three_modes(handle):
         print(f"[gemm] {name}: error / sum|a||b| median {res[name][0]:.3e} p99 {res[name][1]:.3e}; "
               f"max error / max sum|a||b| {res[name][2]:.3e}", flush=True)
         del C, err, rel
-    # split-bf16 recovers fp32 accuracy: its typical error is that of the fp32 product, far below bf16's
-    assert res["split_bf16_3"][1] <= 2.0 * res["fp32"][1] + 1e-9
+    # split-bf16 recovers fp32 accuracy: hi + lo represents each input to 2^-18 relative (the lo part's own rounding),
+    # so its typical error is within a small factor of the fp32 product's (about 2x at K = 8192), its largest error
+    # that of fp32, and both far below bf16's
+    assert res["split_bf16_3"][1] <= 4.0 * res["fp32"][1] + 1e-9
+    assert res["split_bf16_3"][2] <= 2.0 * res["fp32"][2] + 1e-9
     assert res["split_bf16_3"][1] < 0.05 * res["bf16"][1]
 
 
