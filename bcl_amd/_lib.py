@@ -47,6 +47,7 @@ SIGNATURES = [
     ("bclx_synchronize", _I32, [_P]),
     ("bclx_set_allocator", _I32, [_P, _P, _P, _P]),
     ("bclx_spgemm", _I32, [_P, ctypes.POINTER(bclx_csr), ctypes.POINTER(bclx_csr), ctypes.POINTER(bclx_csr)]),
+    ("bclx_spgemm_noalloc", _I32, [_P, ctypes.POINTER(bclx_csr), ctypes.POINTER(bclx_csr), ctypes.POINTER(bclx_csr)]),
     ("bclx_spgemm_symbolic", _I32, [_P, ctypes.POINTER(bclx_csr), ctypes.POINTER(bclx_csr), ctypes.POINTER(_P),
                                     ctypes.POINTER(_I64)]),
     ("bclx_spgemm_numeric", _I32, [_P, ctypes.POINTER(bclx_csr)]),

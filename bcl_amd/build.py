@@ -26,7 +26,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-fvisibility=hidden", "-I" + os.path.join(ROOT, "include"), "-Wno-unused-result",
 ]
-HIP_SOURCES = ["core.hip", "spgemm.hip", "stage.hip", "mlp.hip", "spmm.hip", "gemm.hip", "accumulate.hip", "tiles.hip"]
+HIP_SOURCES = ["core.hip", "spgemm.hip", "stage.hip", "mlp.hip", "spmm.hip", "gemm.hip", "accumulate.hip", "tiles.hip",
+               "noalloc.hip"]
 CPP_SOURCES = ["mmio.cpp"]  # host-only parts of the C ABI (g++)
 
 

@@ -51,6 +51,7 @@ bclx_status bclx_create(int device, void* stream, bclx_handle** out) {
     BCLX_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     h->num_cus = cus > 0 ? cus : 256;
     h->lds_per_block = lds > 0 ? (size_t)lds : 65536;
+    init_noalloc_scratch(device);
     *out = h.release();
   });
 }

@@ -79,18 +79,34 @@ struct DevBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
-// Bump allocator over one device buffer: sub-buffers are 256-byte aligned.
+// Bump allocator over one device buffer: sub-buffers are 256-byte aligned. The buffer is the arena's own (allocated
+// at commit), or caller memory given to use_external (nothing is allocated; commit fails when it is too small).
 struct Arena {
   DevBuf buf;
+  char* ext = nullptr;
+  size_t ext_bytes = 0;
   size_t used = 0;
-  std::vector<std::pair<size_t, size_t>> plan;  // (offset, bytes) of each reservation
   size_t reserve(size_t bytes) {
     size_t off = (used + 255) & ~size_t(255);
     used = off + bytes;
     return off;
   }
-  void commit() { buf.reserve(used); }
-  template <typename T> T* at(size_t off) const { return reinterpret_cast<T*>(static_cast<char*>(buf.p) + off); }
+  void use_external(void* p, size_t bytes) {
+    ext = static_cast<char*>(p);
+    ext_bytes = bytes;
+  }
+  void commit() {
+    if (ext) {
+      BCLX_REQUIRE(used <= ext_bytes, BCLX_ERR_OVERFLOW,
+                   "workspace of " + std::to_string(used) + " bytes exceeds the " + std::to_string(ext_bytes) +
+                       "-byte scratch");
+    } else {
+      buf.reserve(used);
+    }
+  }
+  template <typename T> T* at(size_t off) const {
+    return reinterpret_cast<T*>((ext ? ext : static_cast<char*>(buf.p)) + off);
+  }
 };
 
 }  // namespace bclx
@@ -136,6 +152,10 @@ inline void set_device(bclx_handle* h) {
 }
 
 inline unsigned div_up(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Looks up, once per device and with that device current, the address of the fixed scratch array that
+// bclx_spgemm_noalloc works in.
+void init_noalloc_scratch(int device);
 
 // Zeroed heavy-row accumulators: nslots x ndoubles, grown on demand, zeroed once when (re)allocated.
 void ensure_dense_slots(bclx_handle* h, int nslots, size_t ndoubles);

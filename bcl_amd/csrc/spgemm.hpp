@@ -61,8 +61,12 @@ struct TwoPhase {
   FlatRows<T, RP> F;
   int64_t nnz_c = -1;
   int64_t m = 0;
-  TwoPhase(bclx_handle* h_, const TileDev<T, RP>& tile, const BinTuning& tn) : h(h_) {
+  // scratch / scratch_bytes: device memory the workspace is laid out in instead of being allocated
+  TwoPhase(bclx_handle* h_, const TileDev<T, RP>& tile, const BinTuning& tn, void* scratch = nullptr,
+           size_t scratch_bytes = 0)
+      : h(h_) {
     std::vector<TileDev<T, RP>> tiles{tile};
+    if (scratch) F.arena.use_external(scratch, scratch_bytes);
     F.setup(h, tiles, false);
     F.R.tn = tn;
     m = tile.m;

@@ -97,6 +97,15 @@ BCLX_API bclx_status bclx_spgemm_plan_destroy(bclx_spgemm_plan* plan);
  * (nnz entries each, at least one element) are allocated with the handle's allocator and owned by the caller, who
  * releases them with the matching free function (hipFree by default). Sets C->m, C->n, C->nnz and C->dtype. */
 BCLX_API bclx_status bclx_spgemm(bclx_handle* h, const bclx_csr* A, const bclx_csr* B, bclx_csr* C);
+/* C = A * B in one call that allocates no device memory, for callers that cannot call hipMalloc (code under HIP
+ * graph capture, serving loops with fixed memory, harnesses that forbid allocation). No counterpart in the reference.
+ * C->row_ptr (A->m + 1 entries of C->ptr_width), C->col_idx and C->vals are the caller's; on entry C->nnz is how many
+ * entries they hold, on return nnz(C). A result that does not fit fails with BCLX_ERR_OVERFLOW and C->nnz set to the
+ * entries needed. The per-row workspace lives in a fixed 64 MiB device array that each GPU holds until the process
+ * ends: A may have about 600 K rows at most (more fails with BCLX_ERR_OVERFLOW), and B at most 2^19 columns (more
+ * fails with BCLX_ERR_UNSUPPORTED). Calls that overlap on one GPU share that array and race. Synchronises the
+ * handle's stream. */
+BCLX_API bclx_status bclx_spgemm_noalloc(bclx_handle* h, const bclx_csr* A, const bclx_csr* B, bclx_csr* C);
 /* Per-row products (int64, m entries) and, after symbolic, nnz(C_i) (int32) of a plan, copied to host. */
 BCLX_API bclx_status bclx_spgemm_plan_rows(bclx_spgemm_plan* plan, int64_t* h_products, int32_t* h_row_nnz);
 
