@@ -67,6 +67,9 @@ SIGNATURES = [
     ("bclx_mlp_adam", _I32, [_P, _P, _P, _P, _P, _I32, _F32, _F32, _F32, _F32]),
     ("bclx_spmm", _I32, [_P, ctypes.POINTER(bclx_csr), _P, _I32, _I64, _P, _I64, _I64, _F32, _F32]),
     ("bclx_gemm", _I32, [_P, _I32, _I32, _I64, _I64, _I64, _F32, _P, _I64, _P, _I64, _F32, _P, _I64, _I32]),
+    ("bclx_spmm_f64", _I32, [_P, ctypes.POINTER(bclx_csr), _P, _I64, _P, _I64, _I64, ctypes.c_double, ctypes.c_double]),
+    ("bclx_gemm_f64", _I32, [_P, _I32, _I32, _I64, _I64, _I64, ctypes.c_double, _P, _I64, _P, _I64, ctypes.c_double, _P,
+                             _I64]),
     ("bclx_gemm_f64_ref", _I32, [_P, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64]),
     ("bclx_csr_split_grid", _I32, [_P, ctypes.POINTER(bclx_csr), _I64, _I64, _I32, ctypes.POINTER(bclx_csr)]),
     ("bclx_dense_binary_op", _I32, [_P, _I32, _I32, _I64, _I64, _P, _I64, _P, _I64, _P, _I64]),

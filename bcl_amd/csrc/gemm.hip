@@ -2,7 +2,7 @@
 // calls through cblas_gemm_wrapper_ (bcl/containers/algorithms/cblas_wrapper.hpp:16-40, called with
 // alpha = beta = 1 from gemm_cowns, bcl/containers/algorithms/gemm.hpp:86-107).
 //
-// Three arithmetic modes on the matrix cores:
+// Three arithmetic modes on the matrix cores for fp32 operands (bclx_gemm), and an fp64 product (bclx_gemm_f64):
 //   0  FP32: fp32 inputs, v_mfma_f32_32x32x2f32 (exact fp32 products, fp32 accumulation)
 //   1  BF16: inputs rounded to bf16 (round to nearest even), v_mfma_f32_32x32x16_bf16, fp32 accumulation
 //   2  split-BF16 3-term: x = x_hi + x_lo with x_hi = bf16(x), x_lo = bf16(x - x_hi); a*b is taken as
@@ -243,9 +243,148 @@ __global__ __launch_bounds__(kGThreads, TM * TN > 4 ? 2 : 1) void k_gemm(const t
     }
 }
 
+// ---- FP64 GEMM on the matrix cores (cblas_gemm_wrapper_'s double overload -> cblas_dgemm,
+// bcl/containers/algorithms/cblas_wrapper.hpp:29-40) ------------------------------------------------------------
+// Packed operands as above (op(A) as [M'][K'], op(B) as [N'][K'], K contiguous, zero-padded), block tile 128 x 128
+// of four wavefronts (2 x 2), each owning 4 x 4 tiles of 16 x 16 computed by v_mfma_f64_16x16x4f64 (fp64 products
+// and sums). K tiles of 16 doubles are staged through LDS; the next one is loaded into registers meanwhile.
+
+__global__ __launch_bounds__(256) void k_pack_f64(const double* __restrict__ src, int64_t R, int64_t K, int64_t rs,
+                                                  int64_t cs, double* dst, int64_t Rp, int64_t Kp) {
+  __shared__ double tile[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  const bool rows_contig = cs != 1;
+  for (int q = ty; q < 32; q += 8) {
+    int64_t r, c;
+    if (rows_contig) { r = r0 + tx; c = c0 + q; }
+    else { r = r0 + q; c = c0 + tx; }
+    const double v = (r < R && c < K) ? src[r * rs + c * cs] : 0.0;
+    if (rows_contig) tile[tx][q] = v;
+    else tile[q][tx] = v;
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += 8) {
+    const int64_t r = r0 + q, c = c0 + tx;
+    if (r < Rp && c < Kp) dst[r * Kp + c] = tile[q][tx];
+  }
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+constexpr int kG64BK = 16;                // K per LDS tile
+constexpr int kG64LDK = kG64BK + 2;       // padded row (bank spread)
+constexpr int kG64T = 4;                  // 16 x 16 MFMA tiles per wavefront side
+constexpr int kG64BM = 2 * 16 * kG64T;    // 128
+
+__global__ __launch_bounds__(kGThreads) void k_gemm_f64(const double* __restrict__ Ap, const double* __restrict__ Bp,
+                                                        int64_t M, int64_t N, int64_t Kp, double alpha, double beta,
+                                                        double* C, int64_t ldc, int tiles_m, int tiles_n) {
+  __shared__ __align__(16) double As[kG64BM * kG64LDK];
+  __shared__ __align__(16) double Bs[kG64BM * kG64LDK];
+  const int T = tiles_m * tiles_n;
+  int b = blockIdx.x;
+  if (T % 8 == 0) b = (b % 8) * (T / 8) + (b / 8);  // XCD-aware order, as in k_gemm
+  const int tm = b / tiles_n, tn = b % tiles_n;
+  const int64_t m0 = (int64_t)tm * kG64BM, n0 = (int64_t)tn * kG64BM;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  constexpr int VPR = kG64BK / 2;                 // 16-byte vectors per tile row
+  constexpr int VPT = kG64BM * VPR / kGThreads;   // per thread, per operand
+  const double* __restrict__ Ag = Ap + m0 * Kp;
+  const double* __restrict__ Bg = Bp + n0 * Kp;
+  f64x2 ra[VPT], rb[VPT];
+  auto gload = [&](int64_t k0) {
+    static_for<0, VPT>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int v = tid + i * kGThreads;
+      const int64_t o = (int64_t)(v / VPR) * Kp + k0 + (v % VPR) * 2;
+      ra[i] = *reinterpret_cast<const f64x2*>(Ag + o);
+      rb[i] = *reinterpret_cast<const f64x2*>(Bg + o);
+    });
+  };
+  auto lstore = [&]() {
+    static_for<0, VPT>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int v = tid + i * kGThreads;
+      const int o = (v / VPR) * kG64LDK + (v % VPR) * 2;
+      *reinterpret_cast<f64x2*>(&As[o]) = ra[i];
+      *reinterpret_cast<f64x2*>(&Bs[o]) = rb[i];
+    });
+  };
+  f64x4 acc[kG64T][kG64T];
+#pragma unroll
+  for (int a = 0; a < kG64T; a++)
+#pragma unroll
+    for (int c = 0; c < kG64T; c++) acc[a][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+  const int nk = (int)(Kp / kG64BK);
+  gload(0);
+  lstore();
+  __syncthreads();
+  const int fr = lane & 15, fk = lane >> 4;  // operand lane layout: row (or column) fr, k offset fk
+  for (int kt = 0; kt < nk; kt++) {
+    if (kt + 1 < nk) gload((int64_t)(kt + 1) * kG64BK);
+#pragma unroll
+    for (int ks = 0; ks < kG64BK; ks += 4) {
+      double af[kG64T], bf[kG64T];
+#pragma unroll
+      for (int t = 0; t < kG64T; t++) {
+        af[t] = As[(wm * 16 * kG64T + t * 16 + fr) * kG64LDK + ks + fk];
+        bf[t] = Bs[(wn * 16 * kG64T + t * 16 + fr) * kG64LDK + ks + fk];
+      }
+#pragma unroll
+      for (int a = 0; a < kG64T; a++)
+#pragma unroll
+        for (int c = 0; c < kG64T; c++) acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[c], acc[a][c], 0, 0, 0);
+    }
+    __syncthreads();
+    if (kt + 1 < nk) {
+      lstore();
+      __syncthreads();
+    }
+  }
+  // accumulator layout of a 16 x 16 fp64 tile: lane l holds column l % 16, rows l / 16 + 4 * v
+  static_for<0, kG64T>([&](auto ac) {
+    constexpr int a = decltype(ac)::value;
+    static_for<0, kG64T>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      const int64_t col = n0 + wn * 16 * kG64T + c * 16 + fr;
+      static_for<0, 4>([&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        const int64_t row = m0 + wm * 16 * kG64T + a * 16 + fk + 4 * v;
+        if (row < M && col < N) {
+          double* p = C + row * ldc + col;
+          *p = beta == 0.0 ? alpha * acc[a][c][v] : fma(beta, *p, alpha * acc[a][c][v]);
+        }
+      });
+    });
+  });
+}
+
+static void gemm_f64_run(bclx_handle* h, bool ta, bool tb, int64_t m, int64_t n, int64_t k, double alpha,
+                         const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+                         int64_t ldc) {
+  const int64_t Mp = (m + kG64BM - 1) / kG64BM * kG64BM, Np = (n + kG64BM - 1) / kG64BM * kG64BM;
+  const int64_t Kp = std::max<int64_t>(32, (k + 31) / 32 * 32);  // whole pack tiles and whole K tiles
+  const size_t a_bytes = (size_t)Mp * Kp * 8, off_b = (a_bytes + 255) / 256 * 256;
+  h->scratch.reserve(off_b + (size_t)Np * Kp * 8 + 256);
+  double* Ap = h->scratch.as<double>();
+  double* Bp = reinterpret_cast<double*>(h->scratch.as<char>() + off_b);
+  hipLaunchKernelGGL(k_pack_f64, dim3((unsigned)(Kp / 32), (unsigned)(Mp / 32)), dim3(256), 0, h->stream, A, m, k,
+                     ta ? (int64_t)1 : lda, ta ? lda : (int64_t)1, Ap, Mp, Kp);
+  BCLX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pack_f64, dim3((unsigned)(Kp / 32), (unsigned)(Np / 32)), dim3(256), 0, h->stream, B, n, k,
+                     tb ? ldb : (int64_t)1, tb ? (int64_t)1 : ldb, Bp, Np, Kp);
+  BCLX_HIP(hipGetLastError());
+  const int tiles_m = (int)(Mp / kG64BM), tiles_n = (int)(Np / kG64BM);
+  hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)(tiles_m * tiles_n)), dim3(kGThreads), 0, h->stream, Ap, Bp, m, n, Kp,
+                     alpha, beta, C, ldc, tiles_m, tiles_n);
+  BCLX_HIP(hipGetLastError());
+}
+
 // FP64 reference: C += A * B (fp32 inputs widened exactly), 16 x 16 tiles, row-major. Error oracle only.
-__global__ __launch_bounds__(256) void k_gemm_f64(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
-                                                  const float* B, int64_t ldb, double* C, int64_t ldc) {
+__global__ __launch_bounds__(256) void k_gemm_f64_ref(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
+                                                      const float* B, int64_t ldb, double* C, int64_t ldc) {
   __shared__ double as[16][17], bs[16][17];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   const int64_t row = (int64_t)blockIdx.y * 16 + ty, col = (int64_t)blockIdx.x * 16 + tx;
@@ -329,14 +468,29 @@ bclx_status bclx_gemm(bclx_handle* h, int32_t trans_a, int32_t trans_b, int64_t 
   });
 }
 
+bclx_status bclx_gemm_f64(bclx_handle* h, int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
+                          double alpha, const double* d_A, int64_t lda, const double* d_B, int64_t ldb, double beta,
+                          double* d_C, int64_t ldc) {
+  return guarded([&] {
+    BCLX_REQUIRE(h && d_C, BCLX_ERR_INVALID, "null handle or C");
+    BCLX_REQUIRE(m >= 0 && n >= 0 && k >= 0, BCLX_ERR_INVALID, "negative size");
+    BCLX_REQUIRE(ldc >= n, BCLX_ERR_INVALID, "ldc < n");
+    BCLX_REQUIRE(lda >= (trans_a ? m : k) && ldb >= (trans_b ? k : n), BCLX_ERR_INVALID, "lda / ldb too small");
+    BCLX_REQUIRE(k == 0 || (d_A && d_B), BCLX_ERR_INVALID, "null A or B");
+    set_device(h);
+    if (m == 0 || n == 0) return;
+    gemm_f64_run(h, trans_a != 0, trans_b != 0, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
+  });
+}
+
 bclx_status bclx_gemm_f64_ref(bclx_handle* h, int64_t m, int64_t n, int64_t k, const float* d_A, int64_t lda,
                               const float* d_B, int64_t ldb, double* d_C, int64_t ldc) {
   return guarded([&] {
     BCLX_REQUIRE(h && d_A && d_B && d_C && m >= 0 && n >= 0 && k >= 0, BCLX_ERR_INVALID, "bad argument");
     set_device(h);
     if (m == 0 || n == 0) return;
-    hipLaunchKernelGGL(k_gemm_f64, dim3(div_up(n, 16), div_up(m, 16)), dim3(256), 0, h->stream, m, n, k, d_A, lda, d_B,
-                       ldb, d_C, ldc);
+    hipLaunchKernelGGL(k_gemm_f64_ref, dim3(div_up(n, 16), div_up(m, 16)), dim3(256), 0, h->stream, m, n, k, d_A,
+                       lda, d_B, ldb, d_C, ldc);
     BCLX_HIP(hipGetLastError());
   });
 }

@@ -8,8 +8,9 @@
 // VEC consecutive columns of a 64*VEC column pass; for every entry the wavefront gathers one B row segment
 // (coalesced; the loads of eight entries are issued together), multiplies by the entry's value (broadcast from the
 // lane that loaded it) and accumulates in fp32.
-// Each finished row (or a chunk's part of a row cut by a chunk boundary) is added to C with fp32 atomics that return
-// nothing, so no wavefront waits on a read of C. B is fp32 or bf16 (converted on load); accumulation is fp32.
+// Each finished row (or a chunk's part of a row cut by a chunk boundary) is added to C with atomics that return
+// nothing, so no wavefront waits on a read of C. B is fp32 or bf16 (converted on load) with fp32 accumulation, or,
+// for the fp64 product (bclx_spmm_f64, the reference's double spmm_wrapper, spgemm.hpp:119-149), fp64 throughout.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -24,6 +25,14 @@ __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32
 
 template <typename BT, int VEC>
 struct BLoad;
+template <>
+struct BLoad<double, 2> {
+  static __device__ inline void load(const double* p, double* o) {
+    const double2 v = *reinterpret_cast<const double2*>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+  }
+};
 template <>
 struct BLoad<float, 2> {
   static __device__ inline void load(const float* p, float* o) {
@@ -42,7 +51,8 @@ struct BLoad<uint16_t, 2> {
 };
 template <typename BT>
 struct BLoad<BT, 1> {
-  static __device__ inline void load(const BT* p, float* o) {
+  template <typename VT>
+  static __device__ inline void load(const BT* p, VT* o) {
     if constexpr (sizeof(BT) == 2) o[0] = bf16_to_f32(*p);
     else o[0] = *p;
   }
@@ -50,18 +60,19 @@ struct BLoad<BT, 1> {
 
 // C segment += alpha * acc with fp32 atomics that return nothing: the wavefront never waits on a read of C, and a row
 // split between chunks sums its parts in L2.
-template <int VEC>
-__device__ inline void spmm_flush(float* cr, const float* acc, float alpha, int left) {
+template <int VEC, typename VT>
+__device__ inline void spmm_flush(VT* cr, const VT* acc, VT alpha, int left) {
 #pragma unroll
   for (int v = 0; v < VEC; v++)
     if (v < left) __hip_atomic_fetch_add(&cr[v], alpha * acc[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename RP, typename BT, int VEC>
+// VT: A's values, the accumulation and C (float, or double for the fp64 product); BT: B (VT, or bf16 under float).
+template <typename RP, typename VT, typename BT, int VEC>
 __global__ __launch_bounds__(64 * kSpmmWaves) void k_spmm(const RP* __restrict__ rp, const int* __restrict__ col,
-                                                          const float* __restrict__ val, int64_t m, int64_t nnz,
-                                                          const BT* __restrict__ B, int64_t ldb, float* C, int64_t ldc,
-                                                          int n, float alpha) {
+                                                          const VT* __restrict__ val, int64_t m, int64_t nnz,
+                                                          const BT* __restrict__ B, int64_t ldb, VT* C, int64_t ldc,
+                                                          int n, VT alpha) {
   const int lane = threadIdx.x & 63;
   const int64_t chunk = (int64_t)blockIdx.x * kSpmmWaves + (threadIdx.x >> 6);
   const int64_t e0 = chunk * kSpmmChunk;
@@ -81,26 +92,26 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void k_spmm(const RP* __restrict__
     const bool cok = cj < n;
     int64_t r = lo;
     int64_t rs = (int64_t)rp[r], re = __shfl(my_re, 0, 64);
-    float acc[VEC];
+    VT acc[VEC];
 #pragma unroll
-    for (int v = 0; v < VEC; v++) acc[v] = 0.f;
+    for (int v = 0; v < VEC; v++) acc[v] = VT(0);
     for (int64_t eb = e0; eb < e1; eb += 64) {
       // stage 64 entries: lane q holds entry eb + q
       const int64_t eq = eb + lane;
       const int my_col = eq < e1 ? col[eq] : 0;
-      const float my_val = eq < e1 ? val[eq] : 0.f;
+      const VT my_val = eq < e1 ? val[eq] : VT(0);
       const int cnt = (int)((e1 - eb) < 64 ? (e1 - eb) : 64);
       for (int q0 = 0; q0 < cnt; q0 += kSpmmUnroll) {
         // issue the B-row loads of kSpmmUnroll entries before using any, so their latencies overlap
-        float b[kSpmmUnroll][VEC];
-        float a[kSpmmUnroll];
+        VT b[kSpmmUnroll][VEC];
+        VT a[kSpmmUnroll];
 #pragma unroll
         for (int u = 0; u < kSpmmUnroll; u++) {
           const int q = q0 + u < cnt ? q0 + u : cnt - 1;
           const int k = __shfl(my_col, q, 64);
           a[u] = __shfl(my_val, q, 64);
 #pragma unroll
-          for (int v = 0; v < VEC; v++) b[u][v] = 0.f;
+          for (int v = 0; v < VEC; v++) b[u][v] = VT(0);
           if (cok && q0 + u < cnt) {
             if (cj + VEC <= n) {
               BLoad<BT, VEC>::load(B + (int64_t)k * ldb + cj, b[u]);
@@ -116,13 +127,13 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void k_spmm(const RP* __restrict__
           while (e >= re) {  // flush the finished row (empty rows have nothing to add), move to the row holding e
             if (cok && re > rs) spmm_flush<VEC>(C + r * ldc + cj, acc, alpha, n - cj);
 #pragma unroll
-            for (int v = 0; v < VEC; v++) acc[v] = 0.f;
+            for (int v = 0; v < VEC; v++) acc[v] = VT(0);
             r++;
             rs = re;
             re = r - lo < 64 ? __shfl(my_re, (int)(r - lo), 64) : (int64_t)rp[r + 1];
           }
 #pragma unroll
-          for (int v = 0; v < VEC; v++) acc[v] = fmaf(a[u], b[u][v], acc[v]);
+          for (int v = 0; v < VEC; v++) acc[v] = fma(a[u], b[u][v], acc[v]);
         }
       }
     }
@@ -130,18 +141,28 @@ __global__ __launch_bounds__(64 * kSpmmWaves) void k_spmm(const RP* __restrict__
   }
 }
 
-__global__ void k_scale_rows(float* C, int64_t m, int64_t n, int64_t ldc, float beta) {
+template <typename VT>
+__global__ void k_scale_rows(VT* C, int64_t m, int64_t n, int64_t ldc, VT beta) {
   const int64_t total = m * n;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t / n, j = t % n;
-    float* p = C + i * ldc + j;
-    *p = beta == 0.f ? 0.f : beta * *p;
+    VT* p = C + i * ldc + j;
+    *p = beta == VT(0) ? VT(0) : beta * *p;
   }
 }
 
-template <typename RP, typename BT>
-static void spmm_impl(bclx_handle* h, const bclx_csr* A, const void* Bv, int64_t ldb, float* C, int64_t ldc,
-                      int64_t n, float alpha) {
+template <typename VT>
+static void scale_rows(bclx_handle* h, VT* C, int64_t m, int64_t n, int64_t ldc, VT beta) {
+  if (beta == VT(1) || m == 0 || n == 0) return;
+  const int64_t total = m * n;
+  hipLaunchKernelGGL((k_scale_rows<VT>), dim3((unsigned)std::min<int64_t>((total + 255) / 256, 65536)), dim3(256), 0,
+                     h->stream, C, m, n, ldc, beta);
+  BCLX_HIP(hipGetLastError());
+}
+
+template <typename RP, typename VT, typename BT>
+static void spmm_impl(bclx_handle* h, const bclx_csr* A, const void* Bv, int64_t ldb, VT* C, int64_t ldc, int64_t n,
+                      VT alpha) {
   const RP* rp = static_cast<const RP*>(A->row_ptr);
   const BT* B = static_cast<const BT*>(Bv);
   const int64_t chunks = (A->nnz + kSpmmChunk - 1) / kSpmmChunk;
@@ -150,11 +171,11 @@ static void spmm_impl(bclx_handle* h, const bclx_csr* A, const void* Bv, int64_t
   const bool vec2 = (n % 2 == 0) && (ldb % 2 == 0) && (ldc % 2 == 0) &&
                     ((reinterpret_cast<uintptr_t>(B) % (2 * sizeof(BT))) == 0);
   if (vec2)
-    hipLaunchKernelGGL((k_spmm<RP, BT, 2>), dim3(grid), dim3(64 * kSpmmWaves), 0, h->stream, rp, A->col_idx,
-                       static_cast<const float*>(A->vals), A->m, A->nnz, B, ldb, C, ldc, (int)n, alpha);
+    hipLaunchKernelGGL((k_spmm<RP, VT, BT, 2>), dim3(grid), dim3(64 * kSpmmWaves), 0, h->stream, rp, A->col_idx,
+                       static_cast<const VT*>(A->vals), A->m, A->nnz, B, ldb, C, ldc, (int)n, alpha);
   else
-    hipLaunchKernelGGL((k_spmm<RP, BT, 1>), dim3(grid), dim3(64 * kSpmmWaves), 0, h->stream, rp, A->col_idx,
-                       static_cast<const float*>(A->vals), A->m, A->nnz, B, ldb, C, ldc, (int)n, alpha);
+    hipLaunchKernelGGL((k_spmm<RP, VT, BT, 1>), dim3(grid), dim3(64 * kSpmmWaves), 0, h->stream, rp, A->col_idx,
+                       static_cast<const VT*>(A->vals), A->m, A->nnz, B, ldb, C, ldc, (int)n, alpha);
   BCLX_HIP(hipGetLastError());
 }
 
@@ -174,19 +195,29 @@ bclx_status bclx_spmm(bclx_handle* h, const bclx_csr* A, const void* d_B, int32_
     BCLX_REQUIRE(n >= 0 && ldb >= n && ldc >= n, BCLX_ERR_INVALID, "need ldb >= n and ldc >= n");
     BCLX_REQUIRE(A->nnz == 0 || (A->col_idx && A->vals && d_B), BCLX_ERR_INVALID, "null A arrays or B");
     set_device(h);
-    if (beta != 1.f && A->m > 0 && n > 0) {
-      const int64_t total = A->m * n;
-      hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 65536)), dim3(256), 0,
-                         h->stream, d_C, A->m, n, ldc, beta);
-      BCLX_HIP(hipGetLastError());
-    }
+    scale_rows<float>(h, d_C, A->m, n, ldc, beta);
     if (A->ptr_width == 8) {
-      if (b_dtype == BCLX_F32) spmm_impl<int64_t, float>(h, A, d_B, ldb, d_C, ldc, n, alpha);
-      else spmm_impl<int64_t, uint16_t>(h, A, d_B, ldb, d_C, ldc, n, alpha);
+      if (b_dtype == BCLX_F32) spmm_impl<int64_t, float, float>(h, A, d_B, ldb, d_C, ldc, n, alpha);
+      else spmm_impl<int64_t, float, uint16_t>(h, A, d_B, ldb, d_C, ldc, n, alpha);
     } else {
-      if (b_dtype == BCLX_F32) spmm_impl<int32_t, float>(h, A, d_B, ldb, d_C, ldc, n, alpha);
-      else spmm_impl<int32_t, uint16_t>(h, A, d_B, ldb, d_C, ldc, n, alpha);
+      if (b_dtype == BCLX_F32) spmm_impl<int32_t, float, float>(h, A, d_B, ldb, d_C, ldc, n, alpha);
+      else spmm_impl<int32_t, float, uint16_t>(h, A, d_B, ldb, d_C, ldc, n, alpha);
     }
+  });
+}
+
+bclx_status bclx_spmm_f64(bclx_handle* h, const bclx_csr* A, const double* d_B, int64_t ldb, double* d_C, int64_t ldc,
+                          int64_t n, double alpha, double beta) {
+  return guarded([&] {
+    BCLX_REQUIRE(h && A && d_C, BCLX_ERR_INVALID, "null argument");
+    BCLX_REQUIRE(A->dtype == BCLX_F64, BCLX_ERR_INVALID, "A must hold fp64 values");
+    BCLX_REQUIRE(A->ptr_width == 4 || A->ptr_width == 8, BCLX_ERR_INVALID, "A->ptr_width must be 4 or 8");
+    BCLX_REQUIRE(n >= 0 && ldb >= n && ldc >= n, BCLX_ERR_INVALID, "need ldb >= n and ldc >= n");
+    BCLX_REQUIRE(A->nnz == 0 || (A->col_idx && A->vals && d_B), BCLX_ERR_INVALID, "null A arrays or B");
+    set_device(h);
+    scale_rows<double>(h, d_C, A->m, n, ldc, beta);
+    if (A->ptr_width == 8) spmm_impl<int64_t, double, double>(h, A, d_B, ldb, d_C, ldc, n, alpha);
+    else spmm_impl<int32_t, double, double>(h, A, d_B, ldb, d_C, ldc, n, alpha);
   });
 }
 

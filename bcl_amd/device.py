@@ -113,11 +113,15 @@ def spgemm(h: Handle, A: DeviceCSR, B: DeviceCSR, ptr_dtype=torch.int64, return_
 def spmm(h: Handle, A: DeviceCSR, B: torch.Tensor, C: torch.Tensor, n: int | None = None, alpha: float = 1.0,
          beta: float = 1.0) -> torch.Tensor:
     """C = beta * C + alpha * A @ B[:, :n] in place (BCL's spmm_wrapper is alpha = beta = 1). B row-major fp32 or
-    bf16 (k x ldb), C row-major fp32 (m x ldc)."""
+    bf16 (k x ldb), C row-major fp32 (m x ldc); or A, B and C all fp64 (bclx_spmm_f64)."""
     assert B.dim() == 2 and C.dim() == 2 and B.stride(1) == 1 and C.stride(1) == 1
-    assert C.dtype == torch.float32 and B.dtype in (torch.float32, torch.bfloat16)
     n = B.shape[1] if n is None else n
     d = A.desc()
+    if C.dtype == torch.float64:  # the fp64 product: A, B and C all fp64
+        assert B.dtype == torch.float64 and A.vals.dtype == torch.float64
+        call("bclx_spmm_f64", h.h, ctypes.byref(d), _ptr(B), B.stride(0), _ptr(C), C.stride(0), n, alpha, beta)
+        return C
+    assert C.dtype == torch.float32 and B.dtype in (torch.float32, torch.bfloat16)
     bt = _lib.BCLX_F32 if B.dtype == torch.float32 else _lib.BCLX_BF16
     call("bclx_spmm", h.h, ctypes.byref(d), _ptr(B), bt, B.stride(0), _ptr(C), C.stride(0), n, alpha, beta)
     return C
@@ -199,6 +203,20 @@ def gemm(h: Handle, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, mode: int
     return C
 
 
+def gemm_f64(h: Handle, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, trans_a: bool = False,
+             trans_b: bool = False, alpha: float = 1.0, beta: float = 1.0) -> torch.Tensor:
+    """C = beta * C + alpha * op(A) @ op(B) in place, row-major fp64 on the fp64 matrix cores (cblas_dgemm's place)."""
+    for t in (A, B, C):
+        assert t.dim() == 2 and t.dtype == torch.float64
+        assert t.numel() == 0 or t.stride(1) == 1 or t.shape[1] <= 1, "rows must be contiguous"
+    m, k = (A.shape[1], A.shape[0]) if trans_a else (A.shape[0], A.shape[1])
+    kb, n = (B.shape[1], B.shape[0]) if trans_b else (B.shape[0], B.shape[1])
+    assert k == kb and C.shape[0] >= m and C.shape[1] >= n
+    call("bclx_gemm_f64", h.h, int(trans_a), int(trans_b), m, n, k, alpha, _ptr(A), max(A.stride(0), A.shape[1]),
+         _ptr(B), max(B.stride(0), B.shape[1]), beta, _ptr(C), C.stride(0))
+    return C
+
+
 def gemm_f64_ref(h: Handle, A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     """float64 A @ B on the GPU (fp32 inputs widened exactly): the error oracle for large dense products."""
     m, k = A.shape
@@ -213,4 +231,4 @@ def version() -> str:
 
 
 __all__ = ["Handle", "DeviceCSR", "spgemm", "spmm", "accumulate", "split_grid", "dense_binary_op", "gemm",
-           "gemm_f64_ref", "version", "_lib"]
+           "gemm_f64", "gemm_f64_ref", "version", "_lib"]

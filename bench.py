@@ -251,9 +251,11 @@ def bench_stage(args):
         log(f"cpu baseline ({time.time() - t:.1f}s)")
     del A, B, pairs
     torch.cuda.empty_cache()
-    if not args.headline_only and args.extra_stage_k >= 0:
-        out[f"stage_k{args.extra_stage_k}"] = bench_other_stage(args, h, tiles, args.extra_stage_k, params, margin,
-                                                                slack, tune)
+    if not args.headline_only and args.extra_stages:
+        # the heavier stages of the same grid (k = 1, 2, 4: one high bit in the tile index, about nine times the
+        # products of k = 3); k = 0 (about 3e10 output entries, 240 GB) does not fit one GPU next to its inputs
+        for k in (int(x) for x in args.extra_stages.split(",")):
+            out[f"stage_k{k}"] = bench_other_stage(args, h, tiles, k, params, margin, slack, tune)
     return out
 
 
@@ -506,7 +508,8 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--grid", type=int, default=8)
     ap.add_argument("--stage-k", type=int, default=3)
-    ap.add_argument("--extra-stage-k", type=int, default=2, help="a second stage timed beside the headline (-1: none)")
+    ap.add_argument("--extra-stages", default="2,1,4",
+                    help="other stages of the grid timed beside the headline (comma list; empty: none)")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-exact", action="store_true", help="skip the batched exact and without-network runs")

@@ -178,6 +178,11 @@ BCLX_API bclx_status bclx_mlp_adam(bclx_handle* h, float* d_params, const float*
 BCLX_API bclx_status bclx_spmm(bclx_handle* h, const bclx_csr* A, const void* d_B, int32_t b_dtype, int64_t ldb,
                                float* d_C, int64_t ldc, int64_t n, float alpha, float beta);
 
+/* The same for fp64: A fp64 CSR, B and C row-major fp64, accumulated in fp64 (the reference's spmm_wrapper on
+ * double, spgemm.hpp:119-149). */
+BCLX_API bclx_status bclx_spmm_f64(bclx_handle* h, const bclx_csr* A, const double* d_B, int64_t ldb, double* d_C,
+                                   int64_t ldc, int64_t n, double alpha, double beta);
+
 /* ---- dense GEMM: C = beta * C + alpha * op(A) * op(B), row-major (BCL calls it with alpha = beta = 1) --------
  * mode 0: fp32 inputs, FP32 MFMA (xf32-free, exact fp32 products); mode 1: inputs rounded to bf16, BF16 MFMA
  * with FP32 accumulation; mode 2: split-bf16 3-term (a_hi b_hi + a_hi b_lo + a_lo b_hi) on BF16 MFMA with FP32
@@ -185,6 +190,11 @@ BCLX_API bclx_status bclx_spmm(bclx_handle* h, const bclx_csr* A, const void* d_
 BCLX_API bclx_status bclx_gemm(bclx_handle* h, int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
                                float alpha, const float* d_A, int64_t lda, const float* d_B, int64_t ldb, float beta,
                                float* d_C, int64_t ldc, int32_t mode);
+/* The same for fp64 operands (cblas_gemm_wrapper_'s double overload -> cblas_dgemm,
+ * bcl/containers/algorithms/cblas_wrapper.hpp:29-40): C = beta * C + alpha * op(A) * op(B), row-major, fp64 MFMA. */
+BCLX_API bclx_status bclx_gemm_f64(bclx_handle* h, int32_t trans_a, int32_t trans_b, int64_t m, int64_t n, int64_t k,
+                                   double alpha, const double* d_A, int64_t lda, const double* d_B, int64_t ldb,
+                                   double beta, double* d_C, int64_t ldc);
 /* FP64 dense GEMM C = A * B + C (fp32 inputs widened exactly, fp64 C): the GPU error oracle for large sizes. */
 BCLX_API bclx_status bclx_gemm_f64_ref(bclx_handle* h, int64_t m, int64_t n, int64_t k, const float* d_A,
                                        int64_t lda, const float* d_B, int64_t ldb, double* d_C, int64_t ldc);

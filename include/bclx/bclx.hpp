@@ -9,8 +9,9 @@
 //   a.dot(b), bclx::dot(a, b)    CSRMatrix::dot            CSRMatrix.hpp:229-235
 //   (bclx::dot, bclx::gemm, bclx::split_grid and bclx::SparseHashAccumulator also take BCL::CSRMatrix<T, I, A>
 //   itself, or any type with its public members and constructor, and return that type)
-//   bclx::spmm_wrapper(...)      BCL::spmm_wrapper         spgemm.hpp:140-149 (c += a * b, row-major, ldb / ldc)
-//   bclx::cblas_gemm_wrapper_()  BCL::experimental::cblas_gemm_wrapper_  cblas_wrapper.hpp:16-40
+//   bclx::spmm_wrapper(...)      BCL::spmm_wrapper         spgemm.hpp:140-149 (c += a * b, row-major, ldb / ldc;
+//                                float or double)
+//   bclx::cblas_gemm_wrapper_()  BCL::experimental::cblas_gemm_wrapper_  cblas_wrapper.hpp:16-40 (float and double)
 //   bclx::SparseHashAccumulator  BCL::SparseHashAccumulator  SparseHashAccumulator.hpp:86-183
 //   CSRMatrix::read_MatrixMarket / write_MatrixMarket, CSRMatrix(fname, format), matrix_io::matrix_info
 //                                CSRMatrix.hpp:191-224, :369-411, :532-548; matrix_io.hpp:10-98, :221-357
@@ -565,18 +566,22 @@ struct SparseHashAccumulator {
   }
 };
 
-// BCL::spmm_wrapper: c[i*ldc + j] += sum_k a[i,k] * b[k*ldb + j], j < n; host arrays, fp32 values.
-template <typename I>
-void spmm_wrapper(size_t m, size_t n, size_t k, size_t nnz, float* c, float* b, float* a_values, I* a_rowptr,
-                  I* a_colind, size_t ldb, size_t ldc) {
+// BCL::spmm_wrapper (spgemm.hpp:140-149): c[i*ldc + j] += sum_k a[i,k] * b[k*ldb + j], j < n; host arrays, T = float
+// (fp32 accumulation) or double (fp64 throughout, the reference's double path, spgemm.hpp:119-138).
+template <typename I, typename T>
+void spmm_wrapper(size_t m, size_t n, size_t k, size_t nnz, T* c, T* b, T* a_values, I* a_rowptr, I* a_colind,
+                  size_t ldb, size_t ldc) {
   bclx_handle* h = detail::ctx().handle();
   detail::Dev<I> rp(a_rowptr, m + 1);
   std::vector<int32_t> ci(a_colind, a_colind + nnz);
   detail::Dev<int32_t> col(ci.data(), nnz);
-  detail::Dev<float> val(a_values, nnz);
-  detail::Dev<float> B(b, k * ldb), C(c, m * ldc);
-  bclx_csr A{(int64_t)m, (int64_t)k, (int64_t)nnz, detail::width_of<I>(), BCLX_F32, rp.p, col.p, val.p};
-  detail::check(bclx_spmm(h, &A, B.p, BCLX_F32, (int64_t)ldb, C.p, (int64_t)ldc, (int64_t)n, 1.f, 1.f));
+  detail::Dev<T> val(a_values, nnz);
+  detail::Dev<T> B(b, k * ldb), C(c, m * ldc);
+  bclx_csr A{(int64_t)m, (int64_t)k, (int64_t)nnz, detail::width_of<I>(), detail::dtype_of<T>(), rp.p, col.p, val.p};
+  if constexpr (std::is_same<T, double>::value)
+    detail::check(bclx_spmm_f64(h, &A, B.p, (int64_t)ldb, C.p, (int64_t)ldc, (int64_t)n, 1.0, 1.0));
+  else
+    detail::check(bclx_spmm(h, &A, B.p, BCLX_F32, (int64_t)ldb, C.p, (int64_t)ldc, (int64_t)n, 1.f, 1.f));
   detail::check(bclx_synchronize(h));
   C.to_host(c, m * ldc);
 }
@@ -599,6 +604,26 @@ inline void cblas_gemm_wrapper_(int layout, int transa, int transb, int m, int n
   const size_t a_rows = ta ? k : m, b_rows = tb ? n : k;
   detail::Dev<float> A(a, a_rows * (size_t)lda), B(b, b_rows * (size_t)ldb), C(c, (size_t)m * ldc);
   detail::check(bclx_gemm(h, ta, tb, m, n, k, alpha, A.p, lda, B.p, ldb, beta, C.p, ldc, detail::ctx().gemm_mode));
+  detail::check(bclx_synchronize(h));
+  C.to_host(c, (size_t)m * ldc);
+}
+
+// BCL::experimental::cblas_gemm_wrapper_ (double -> cblas_dgemm, cblas_wrapper.hpp:29-40): fp64 MFMA.
+inline void cblas_gemm_wrapper_(int layout, int transa, int transb, int m, int n, int k, double alpha, const double* a,
+                                int lda, const double* b, int ldb, double beta, double* c, int ldc) {
+  bclx_handle* h = detail::ctx().handle();
+  const bool ta = transa != NoTrans, tb = transb != NoTrans;
+  if (layout == ColMajor) {
+    std::swap(m, n);
+    std::swap(a, b);
+    std::swap(lda, ldb);
+    const bool t = ta;
+    return cblas_gemm_wrapper_(RowMajor, tb ? Trans : NoTrans, t ? Trans : NoTrans, m, n, k, alpha, a, lda, b, ldb,
+                               beta, c, ldc);
+  }
+  const size_t a_rows = ta ? k : m, b_rows = tb ? n : k;
+  detail::Dev<double> A(a, a_rows * (size_t)lda), B(b, b_rows * (size_t)ldb), C(c, (size_t)m * ldc);
+  detail::check(bclx_gemm_f64(h, ta, tb, m, n, k, alpha, A.p, lda, B.p, ldb, beta, C.p, ldc));
   detail::check(bclx_synchronize(h));
   C.to_host(c, (size_t)m * ldc);
 }

@@ -6,6 +6,8 @@
 //   <dir>/dense.bin (m, n, k, then A m x k, B k x n, C m x n as float) -> gemm_out.bin = C + A * B (cblas wrapper,
 //   row-major), gemm_cm.bin = the same product through the column-major path
 //   spmm: spmm_in.bin (n, ld, then B k x ld, C m x ld) with a -> spmm_out.bin (C += a * B[:, :n])
+//   fp64: dense64.bin (m, n, k, then A, B, C as double) -> gemm64_out.bin (row-major result, then column-major);
+//         spmm64_out.bin = C + a * B in fp64 (a's values, B and C of spmm_in.bin widened)
 //   files: a.mtx -> mtx.binary (CSRMatrix(fname) of a MatrixMarket file), mtx_out.mtx (write_MatrixMarket);
 //   a.binary memory-mapped -> slice.binary = rows [100, 700) x columns [50, 1500); split_grid(a, 300, 500) ->
 //   split_<t>.binary; ew.bin (m, n, ld, then X, Y m x ld) -> ew_out.bin = X + Y, X - Y, X * Y, X / Y, then X += Y
@@ -131,6 +133,47 @@ int main(int argc, char** argv) {
     o = fopen((d + "/spmm_out.bin").c_str(), "wb");
     fwrite(SC.data(), sizeof(float), SC.size(), o);
     fclose(o);
+
+    {  // the double overloads: cblas_gemm_wrapper_ (row- and column-major) and spmm_wrapper<int, double>
+      f = fopen((d + "/dense64.bin").c_str(), "rb");
+      auto dd = read_vec<int64_t>(f, 3);
+      const int m6 = (int)dd[0], n6 = (int)dd[1], k6 = (int)dd[2];
+      auto A6 = read_vec<double>(f, (size_t)m6 * k6), B6 = read_vec<double>(f, (size_t)k6 * n6);
+      auto C6 = read_vec<double>(f, (size_t)m6 * n6);
+      fclose(f);
+      std::vector<double> R6 = C6;
+      bclx::cblas_gemm_wrapper_(bclx::RowMajor, bclx::NoTrans, bclx::NoTrans, m6, n6, k6, 1.0, A6.data(), k6,
+                                B6.data(), n6, 1.0, R6.data(), n6);
+      std::vector<double> At6((size_t)k6 * m6), Bt6((size_t)n6 * k6), Ct6((size_t)n6 * m6), M6(C6.size());
+      for (int i = 0; i < m6; i++)
+        for (int q = 0; q < k6; q++) At6[(size_t)q * m6 + i] = A6[(size_t)i * k6 + q];
+      for (int q = 0; q < k6; q++)
+        for (int j = 0; j < n6; j++) Bt6[(size_t)j * k6 + q] = B6[(size_t)q * n6 + j];
+      for (int i = 0; i < m6; i++)
+        for (int j = 0; j < n6; j++) Ct6[(size_t)j * m6 + i] = C6[(size_t)i * n6 + j];
+      bclx::cblas_gemm_wrapper_(bclx::ColMajor, bclx::NoTrans, bclx::NoTrans, m6, n6, k6, 1.0, At6.data(), m6,
+                                Bt6.data(), k6, 1.0, Ct6.data(), m6);
+      for (int i = 0; i < m6; i++)
+        for (int j = 0; j < n6; j++) M6[(size_t)i * n6 + j] = Ct6[(size_t)j * m6 + i];
+      o = fopen((d + "/gemm64_out.bin").c_str(), "wb");
+      fwrite(R6.data(), sizeof(double), R6.size(), o);
+      fwrite(M6.data(), sizeof(double), M6.size(), o);
+      fclose(o);
+      // spmm in fp64 on a with its values widened; B and C from spmm_in.bin widened
+      std::vector<double> av(a.vals_.begin(), a.vals_.end());
+      std::vector<double> B64(SB.begin(), SB.end()), C64(SC.begin(), SC.end());
+      f = fopen((d + "/spmm_in.bin").c_str(), "rb");  // the original C (SC now holds the fp32 result)
+      read_vec<int64_t>(f, 2);
+      read_vec<float>(f, a.n() * ld);
+      auto SC0 = read_vec<float>(f, a.m() * ld);
+      fclose(f);
+      std::copy(SC0.begin(), SC0.end(), C64.begin());
+      bclx::spmm_wrapper<int>(a.m(), sn, a.n(), a.nnz(), C64.data(), B64.data(), av.data(), a.row_ptr_.data(),
+                              a.col_ind_.data(), ld, ld);
+      o = fopen((d + "/spmm64_out.bin").c_str(), "wb");
+      fwrite(C64.data(), sizeof(double), C64.size(), o);
+      fclose(o);
+    }
 
     bclx::CSRMatrix<float, int> mtx(d + "/a.mtx");
     mtx.write_Binary(d + "/mtx.binary");

@@ -44,6 +44,11 @@ def test_cpp_host_layer(tmp_path):
     with open(tmp_path / "dense.bin", "wb") as f:
         np.array([m, n, k], np.int64).tofile(f)
         A.tofile(f), B.tofile(f), C.tofile(f)
+    m6, n6, k6 = 70, 90, 110
+    A6, B6, C6 = (rng.standard_normal(sh) for sh in ((m6, k6), (k6, n6), (m6, n6)))
+    with open(tmp_path / "dense64.bin", "wb") as f:
+        np.array([m6, n6, k6], np.int64).tofile(f)
+        A6.tofile(f), B6.tofile(f), C6.tofile(f)
     sn, ld = 40, 48
     SB = rng.standard_normal((a.n, ld)).astype(np.float32)
     SC = rng.standard_normal((a.m, ld)).astype(np.float32)
@@ -90,6 +95,20 @@ def test_cpp_host_layer(tmp_path):
     mag = np.abs(SC[:, :sn]) + O.spmm_f64(_abs(a), np.ascontiguousarray(np.abs(SB[:, :sn])))
     assert np.all(np.abs(got[:, :sn] - exp) <= 1e-5 * mag + 1e-30)
     np.testing.assert_array_equal(got[:, sn:], SC[:, sn:])
+
+    # fp64 overloads
+    g6 = np.fromfile(tmp_path / "gemm64_out.bin", np.float64).reshape(2, m6, n6)
+    S6 = np.abs(C6) + np.abs(A6) @ np.abs(B6)
+    for q in range(2):
+        assert np.all(np.abs(g6[q] - (C6 + A6 @ B6)) <= 2.0 * (k6 + 2) * 2.0 ** -53 * S6)
+    s6 = np.fromfile(tmp_path / "spmm64_out.bin", np.float64).reshape(a.m, ld)
+    import scipy.sparse
+    a64 = scipy.sparse.csr_matrix((a.vals.astype(np.float64), a.col_idx, a.row_ptr), shape=(a.m, a.n))
+    exp6 = SC[:, :sn].astype(np.float64) + a64 @ SB[:, :sn].astype(np.float64)
+    mag6 = np.abs(SC[:, :sn]) + abs(a64) @ np.abs(SB[:, :sn]).astype(np.float64)
+    L6 = np.diff(a.row_ptr)[:, None] + 2
+    assert np.all(np.abs(s6[:, :sn] - exp6) <= 2 * L6 * 2.0 ** -53 * mag6 + 1e-300)
+    np.testing.assert_array_equal(s6[:, sn:], SC[:, sn:].astype(np.float64))
 
     # files and tiles
     mtx = read_binary(str(tmp_path / "mtx.binary"))

@@ -5,6 +5,8 @@ Error bounds against the float64 oracle, per entry, with S = sum_k |a_ik b_kj|:
   BF16 MFMA        |c - c64| <= 2^-7 * S      (each input rounded to 8 significant bits)
   split-BF16 (3)   |c - c64| <= 2e-5 * S      (dropped lo*lo term and lo rounding ~2^-16, plus fp32 accumulation)
 fill_range inputs (bcl/containers/DMatrix.hpp:589-604, small integers) are exact in every mode.
+The large BF16 cases run the 256 x 128 block tiling (the one the 8192^3 product uses) with every transpose, ragged
+edges and a padded leading dimension.
 """
 import numpy as np
 import pytest
@@ -82,3 +84,25 @@ def test_f64_reference_kernel(handle):
     b = rng.standard_normal((200, 250)).astype(np.float32)
     c = gemm_f64_ref(handle, torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()).cpu().numpy()
     np.testing.assert_allclose(c, O.gemm_f64(a, b), rtol=1e-13, atol=1e-12)
+
+
+@pytest.mark.parametrize("ta,tb,pad", [(False, False, 0), (True, False, 0), (False, True, 0), (True, True, 0),
+                                       (False, False, 3)])
+def test_bf16_fused_large(handle, ta, tb, pad):
+    """4100 x 4130 x 1000 (17 x 17 blocks of 256, ragged in M, N and K), alpha / beta, optional ld padding."""
+    M, N, K = 4100, 4130, 1000
+    g = torch.Generator(device="cuda").manual_seed(M + 2 * ta + tb + pad)
+    rows_a, cols_a = (K, M) if ta else (M, K)
+    rows_b, cols_b = (N, K) if tb else (K, N)
+    A = torch.randn((rows_a, cols_a + pad), device="cuda", generator=g)[:, :cols_a]
+    B = torch.randn((rows_b, cols_b + pad), device="cuda", generator=g)[:, :cols_b]
+    C0 = torch.randn((M, N), device="cuda", generator=g)
+    C = C0.clone()
+    gemm(handle, A, B, C, mode=GEMM_BF16, trans_a=ta, trans_b=tb, alpha=1.25, beta=-0.5)
+    oa = (A.t() if ta else A).contiguous()
+    ob = (B.t() if tb else B).contiguous()
+    ref = gemm_f64_ref(handle, oa, ob) * 1.25 - 0.5 * C0.double()
+    S = gemm_f64_ref(handle, oa.abs(), ob.abs()) * 1.25 + 0.5 * C0.double().abs()
+    worst = float(((C.double() - ref).abs() / S).max())
+    print(f"[gemm] fused bf16 ta={ta} tb={tb} pad={pad}: max |c-c64|/S = {worst:.3e}", flush=True)
+    assert worst <= BOUND[GEMM_BF16]
