@@ -434,6 +434,19 @@ def bench_gemm(args):
                      "rel_err_median": q(e_r, 0.5), "rel_err_p99": q(e_r, 0.99),
                      "error_sample": "2^20 random entries of C; rel_err over entries with |C| >= 0.01 sum|a||b|"}
         del C, d
+    # the fp64 product on the fp64 matrix cores (cblas_dgemm's place), on the same inputs widened exactly
+    from bcl_amd.device import gemm_f64
+    A64, B64 = A.double(), B.double()
+    C = torch.zeros((N, N), device="cuda", dtype=torch.float64)
+    ms, _ = time_steps(lambda: gemm_f64(h, A64, B64, C, beta=0.0), args.steps, args.warmup, h.stream)
+    ms_step = ms / args.steps
+    torch.cuda.synchronize()
+    d = (C - C64).abs()
+    useful = 2.0 * N ** 3 / (ms_step * 1e-3) / 1e12
+    res["fp64"] = {"ms": round(ms_step, 3), "useful_tflops": round(useful, 2), "mfma_util": round(useful / 78.6, 4),
+                   "peak_tflops": 78.6, "max_abs_err_rel_to_max_c": float((d.max() / C64.abs().max()).item()),
+                   "err_over_sum_abs_max": float((d / S).max().item())}
+    del A64, B64, C, d
     f = res["bf16"]
     # CPU: the oracle's float64 GEMM on 16 rows (labelled)
     t = time.perf_counter()
