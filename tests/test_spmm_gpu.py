@@ -71,22 +71,26 @@ def test_rmat_128(handle, bf16):
         assert rel < 1e-2
 
 
-def test_alpha_beta_int64_and_odd_widths(handle):
+@pytest.mark.parametrize("bf16", [False, True])
+def test_alpha_beta_int64_and_odd_widths(handle, bf16):
+    """Widths that take the paired-entry kernel (multiples of 4: 100, 300) and the single-entry one (33)."""
     a = gen.rmat(12, seed=4)
     rng = np.random.default_rng(2)
     b = rng.standard_normal((a.n, 104)).astype(np.float32)
     c0 = rng.standard_normal((a.m, 104)).astype(np.float32)
-    check(handle, a, b, c0, n=100, alpha=2.0, beta=0.5, ptr_dtype=torch.int64)
+    check(handle, a, b, c0, n=100, alpha=2.0, beta=0.5, bf16=bf16, ptr_dtype=torch.int64)
     b = rng.standard_normal((a.n, 33)).astype(np.float32)
     c0 = rng.standard_normal((a.m, 33)).astype(np.float32)
-    check(handle, a, b, c0, alpha=-1.0, beta=0.0)
+    check(handle, a, b, c0, alpha=-1.0, beta=0.0, bf16=bf16)
     b = rng.standard_normal((a.n, 300)).astype(np.float32)
     c0 = rng.standard_normal((a.m, 300)).astype(np.float32)
-    check(handle, a, b, c0)
+    check(handle, a, b, c0, bf16=bf16)
 
 
-def test_long_rows_cross_chunks(handle):
-    """Rows longer than a wavefront's chunk are added with atomics from several chunks."""
+@pytest.mark.parametrize("bf16", [False, True])
+def test_long_rows_cross_chunks(handle, bf16):
+    """Rows longer than a wavefront's chunk are added with atomics from several chunks; rows of one entry and
+    empty rows sit between them, so pairs of entries straddle row ends."""
     m, k = 8, 5000
     rp = np.array([0, 3000, 3000, 3001, 4500, 4500, 4500, 6000, 7000], np.int64)
     rng = np.random.default_rng(3)
@@ -95,7 +99,7 @@ def test_long_rows_cross_chunks(handle):
     a = CSR(m, k, rp, cols, rng.uniform(-1, 1, int(rp[-1])).astype(np.float32))
     b = rng.standard_normal((k, 64)).astype(np.float32)
     c0 = rng.standard_normal((m, 64)).astype(np.float32)
-    check(handle, a, b, c0)
+    check(handle, a, b, c0, bf16=bf16)
 
 
 def test_empty(handle):
