@@ -9,7 +9,7 @@ import pytest
 import torch
 
 import oracle_py as O
-from bcl_amd import gen, predictor
+from bcl_amd import autotune, gen, predictor
 from bcl_amd.csr import CSR
 from bcl_amd.stage import BATCHED_EXACT, LEARNED, PLAIN, StagePlan, stage_tiles
 from conftest import spgemm_value_bound
@@ -163,18 +163,28 @@ def _median_ms(h, plan, reps=7):
     return float(np.median(ts))
 
 
+@pytest.mark.parametrize("sorting", ["off", "shipped"])
 @pytest.mark.parametrize("kind", ["banded", "rmat"])
-def test_predictor_gain(handle, kind):
+def test_predictor_gain(handle, kind, sorting):
     """What the trained network gains over the method without it, on the same stage.
 
     Without the network every row's capacity is the exact bound min(products, n); the exact pipeline runs the
-    symbolic pass instead. All three must give the oracle's result. The trained network must size the slot region
-    well below the bound (its point: smaller tables and output arena) with few overflows; the times are reported.
-    Sorted (ESC) rows are exact in one pass and need no prediction, so sorting is turned off here: every row hashes.
+    symbolic pass instead. All three must give the oracle's result; the times and slot capacities are reported.
+    With sorting off every row hashes and the network sizes all of them: it must size the slot region well below the
+    bound (its point: smaller tables and output arena) with few overflows. With the shipped tuning (sorting on, the
+    autotuned knobs and margin) sorted rows are exact in one pass and the network sizes only the rows beyond the sort
+    bins: its capacities must not exceed the bound.
     """
     params = predictor.load()
     if params is None:
         pytest.skip("no trained predictor weights in bcl_amd/data")
+    if sorting == "shipped":
+        tun, margin, slack = autotune.load_tuning()
+        if tun is None:
+            pytest.skip("no autotuned settings in bcl_amd/data")
+        opts = dict(tuning=tun, margin=margin, slack=slack)
+    else:
+        opts = dict(tuning=np.array([64, 4096, 1, 32, 0]))
     if kind == "banded":
         a = gen.banded(1 << 15, 1 << 15, bw=64, max_per_row=24, seed=11)
         grid, k = 2, 0
@@ -185,8 +195,8 @@ def test_predictor_gain(handle, kind):
     A, B, pairs = stage_tiles(tiles, tiles, grid, k)
     res = {}
     for name, mode, prm in (("trained", LEARNED, params), ("bound", LEARNED, None), ("exact", BATCHED_EXACT, None)):
-        print(f"[gain] {kind} {name}", flush=True)
-        plan = StagePlan(handle, A, B, mode=mode, params=prm, tuning=np.array([64, 4096, 1, 32, 0]))
+        print(f"[gain] {kind} sorting {sorting}: {name}", flush=True)
+        plan = StagePlan(handle, A, B, mode=mode, params=prm, **opts)
         ms = _median_ms(handle, plan)
         st = plan.stats()
         check_stage(plan, pairs)
@@ -195,9 +205,12 @@ def test_predictor_gain(handle, kind):
     t, b, x = res["trained"], res["bound"], res["exact"]
     nnz = t[1]["nnz"]
     assert b[1]["nnz"] == nnz and x[1]["nnz"] == nnz
-    print(f"[gain] {kind}: nnz={nnz} products={t[1]['products']} | trained {t[0]:.3f} ms, slots {t[1]['capacity']}, "
-          f"overflow {t[1]['overflow_rows']}/{t[1]['predicted_rows']} | bound {b[0]:.3f} ms, slots {b[1]['capacity']}"
-          f" | exact {x[0]:.3f} ms", flush=True)
+    print(f"[gain] {kind} sorting {sorting}: nnz={nnz} products={t[1]['products']} | trained {t[0]:.3f} ms, slots "
+          f"{t[1]['capacity']}, overflow {t[1]['overflow_rows']}/{t[1]['predicted_rows']} | bound {b[0]:.3f} ms, "
+          f"slots {b[1]['capacity']} | exact {x[0]:.3f} ms", flush=True)
+    if sorting == "shipped":
+        assert t[1]["capacity"] <= b[1]["capacity"]
+        return
     assert t[1]["predicted_rows"] > 0
     assert t[1]["overflow_rows"] <= 0.02 * t[1]["predicted_rows"] + 8
     assert t[1]["capacity"] < b[1]["capacity"]
