@@ -233,6 +233,19 @@ template <typename T, typename RP>
 void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint) {
   // count_hint >= 0: exact row count known on the host (grid sized to it); < 0: persistent grid.
   if (b == 0 || count_hint == 0) return;
+  if (b == kSmallBin) {
+    unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, div_up(count_hint, 2 * kSmallWaves))
+                                    : (unsigned)(h->num_cus * 8);
+    grid = std::min<unsigned>(grid, 1u << 20);
+    switch (mode) {
+      case 0: hipLaunchKernelGGL((k_esc_small<0, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b); break;
+      case 1: hipLaunchKernelGGL((k_esc_small<1, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b); break;
+      case 2: hipLaunchKernelGGL((k_esc_small<2, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b); break;
+      default: hipLaunchKernelGGL((k_esc_small<3, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b); break;
+    }
+    BCLX_HIP(hipGetLastError());
+    return;
+  }
   if (b >= kEscBin0 && b < kEscBin0 + kEscWaveBins) {
     unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, div_up(count_hint, kEscWaves))
                                     : (unsigned)(h->num_cus * 8);

@@ -14,8 +14,9 @@ namespace bclx {
 
 constexpr int kEmptyKey = 0x7fffffff;  // larger than any column index
 // 0: empty row; 1..9: LDS hash tables; 10: heavy (global accumulator); 11..17: sort-based (ESC) rows of up to
-// 64 * 2^(b - 11) products (bins 11..14: one wavefront per row; 15..17: 2, 4 or 8 wavefronts per row)
-constexpr int kNumBins = 18;
+// 64 * 2^(b - 11) products (bins 11..14: one wavefront per row; 15..17: 2, 4 or 8 wavefronts per row); 18: rows of
+// at most 32 products, sorted in registers by half a wavefront
+constexpr int kNumBins = 19;
 constexpr int kHeavyBin = 10;
 constexpr int kCountStride = 32;  // bin counters sit 128 bytes apart
 constexpr int kEscBin0 = 11;
@@ -23,6 +24,9 @@ constexpr int kEscBins = 7;
 constexpr int kEscWaveBins = 4;                         // bins 11..14 sort within one wavefront
 constexpr int kEscMaxProducts = 64 << (kEscBins - 1);  // 4096
 constexpr int kEscColBits = 19;                         // sort keys pack column (< 2^19) and product index (13 bits)
+constexpr int kSmallBin = 18;
+constexpr int kSmallMax = 32;       // products per row in the small bin
+constexpr int kSmallColBits = 27;   // small-bin keys pack column (< 2^27 - 1) and product index (5 bits)
 constexpr int kRowBlock = 256;         // flat-row padding and analysis block size
 constexpr int kHeavyThreads = 1024;
 
@@ -49,9 +53,12 @@ __host__ __device__ inline int esc_bin(int64_t work) {
   return kEscBin0 + e;
 }
 
-// esc_ok: the tile's columns fit the packed sort key (n <= 2^19).
-__host__ __device__ inline int bin_for(int64_t need, int64_t work, const BinTuning& tn, bool esc_ok = false) {
+// n: the tile's width. Rows of at most 32 products sort in registers (tiles narrower than 2^27 - 1 columns); longer
+// rows sort in the ESC bins when the columns fit their packed key (n <= 2^19). esc_max = 0 turns both off.
+__host__ __device__ inline int bin_for(int64_t need, int64_t work, const BinTuning& tn, int64_t n) {
   if (work <= 0) return 0;
+  if (tn.esc_max > 0 && work <= kSmallMax && n < ((int64_t)1 << kSmallColBits) - 1) return kSmallBin;
+  const bool esc_ok = n <= (1 << kEscColBits);
   if (esc_ok && work > tn.esc_min && work <= tn.esc_max && work <= kEscMaxProducts) return esc_bin(work);
   if (need > tn.heavy_need) return kHeavyBin;
   int64_t slots = need << tn.load_shift;
@@ -252,7 +259,7 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze(RowsDev<T, RP> R, int64_t
     }
     if (in[s]) R.prod[fr] = valid[s] ? prod[s] : 0;
     ub0[s] = prod[s] < (long long)tp[s]->n ? prod[s] : (long long)tp[s]->n;
-    b0[s] = valid[s] ? bin_for(ub0[s], prod[s], R.tn, tp[s]->n <= (1 << kEscColBits)) : 0;
+    b0[s] = valid[s] ? bin_for(ub0[s], prod[s], R.tn, tp[s]->n) : 0;
     if (with_stats && b0[s] == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
     pr[s] = false;
   }
@@ -333,7 +340,7 @@ __global__ __launch_bounds__(kRowBlock) void k_analyze_long(RowsDev<T, RP> R, in
     if (lane == 0) {
       R.prod[fr] = prod;
       const long long ub0 = prod < (long long)t->n ? prod : (long long)t->n;
-      const int b0 = bin_for(ub0, prod, R.tn, t->n <= (1 << kEscColBits));
+      const int b0 = bin_for(ub0, prod, R.tn, t->n);
       if (with_stats) {
         atomicAdd(&R.stats[0], (unsigned long long)prod);
         if (b0 == kHeavyBin) atomicAdd(&R.stats[5], 1ull);
@@ -368,7 +375,7 @@ __global__ __launch_bounds__(kRowBlock) void k_bin(RowsDev<T, RP> R, int src) {
   if (valid) {
     const long long work = R.prod[fr];
     const long long need = src == 0 ? (long long)R.nnz[fr] : (long long)R.cap[fr];
-    b = (need == 0 && src == 0) ? 0 : bin_for(need, work, R.tn, tile.n <= (1 << kEscColBits));
+    b = (need == 0 && src == 0) ? 0 : bin_for(need, work, R.tn, tile.n);
   }
   int pos = 0;
   if (b > 0) pos = atomicAdd(&s_bcount[b], 1);
@@ -1000,6 +1007,152 @@ __global__ __launch_bounds__(64 * kEscWaves) void k_esc(RowsDev<T, RP> R, int bi
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// Rows of at most 32 products, half a wavefront each (two rows per wavefront). The products are expanded one per lane
+// as 32-bit keys (column << 5 | product index), sorted by a bitonic network over the 32 lanes (DPP and swizzle
+// exchanges: no LDS, no barrier), and equal columns are summed in product (A-row) order, in float64, rounded once.
+// A hash row of this size costs a table clear, LDS atomics and a sort of its table in LDS; a one-wavefront ESC row
+// leaves half the lanes idle. Tiles up to 2^27 - 2 columns wide. Modes as in k_esc.
+// ------------------------------------------------------------------------------------------------------------
+
+constexpr int kSmallWaves = 4;  // wavefronts per block
+
+template <int K, int J>
+__device__ inline unsigned small_cx(unsigned key, int hl) {
+  const unsigned other = lane_xor<J>(key, hl);
+  const bool lower = (hl & J) == 0, asc = (hl & K) == 0;
+  return lower == asc ? min(key, other) : max(key, other);
+}
+
+// ascending over the 32 lanes of each half wavefront
+__device__ inline unsigned small_sort(unsigned key, int hl) {
+  key = small_cx<2, 1>(key, hl);
+  key = small_cx<4, 2>(key, hl);
+  key = small_cx<4, 1>(key, hl);
+  key = small_cx<8, 4>(key, hl);
+  key = small_cx<8, 2>(key, hl);
+  key = small_cx<8, 1>(key, hl);
+  key = small_cx<16, 8>(key, hl);
+  key = small_cx<16, 4>(key, hl);
+  key = small_cx<16, 2>(key, hl);
+  key = small_cx<16, 1>(key, hl);
+  key = small_cx<32, 16>(key, hl);
+  key = small_cx<32, 8>(key, hl);
+  key = small_cx<32, 4>(key, hl);
+  key = small_cx<32, 2>(key, hl);
+  key = small_cx<32, 1>(key, hl);
+  return key;
+}
+
+template <int MODE, typename T, typename RP>
+__global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R, int bin) {
+  constexpr bool NUM = MODE != 0;
+  constexpr unsigned kShift = 32 - kSmallColBits;
+  const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
+  const int count = *R.count_of(bin);
+  const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
+  // two rows per wavefront; the loop and every exchange below are wave-uniform (both halves run them)
+  for (int p = (blockIdx.x * kSmallWaves + (threadIdx.x >> 6)) * 2; p < count; p += gridDim.x * kSmallWaves * 2) {
+    const int idx = p + half;
+    const bool active = idx < count;
+    const int fr = active ? items[idx] : 0;
+    TileDev<T, RP> tile;
+    RP a0 = 0, a1 = 0;
+    if (active) {
+      tile = R.tiles[R.blk_tile[fr >> 8]];
+      const int64_t i = fr - tile.flat0;
+      a0 = tile.a_rp[i];
+      a1 = tile.a_rp[i + 1];
+    }
+    const int na = (int)(a1 - a0);
+    const int na_max = max(na, __shfl_xor(na, 32, 64));
+    // expand: product x (A-row order) goes to lane x of the half
+    unsigned key = kEscPad;
+    double pv = 0.0;
+    int base = 0;
+    for (int c0 = 0; c0 < na_max; c0 += 32) {
+      const int ne = min(max(na - c0, 0), 32);
+      int len = 0;
+      RP st = 0;
+      T av = T(0);
+      if (hl < ne) {
+        const int kk = tile.a_col[a0 + c0 + hl];
+        st = tile.b_rp[kk];
+        len = (int)(tile.b_rp[kk + 1] - st);
+        if (NUM) av = tile.a_val[a0 + c0 + hl];
+      }
+      int inc = len;
+#pragma unroll
+      for (int d = 1; d < 32; d <<= 1) {
+        const int t = __shfl_up(inc, d, 32);
+        if (hl >= d) inc += t;
+      }
+      const int ex = inc - len;
+      const int total = __shfl(inc, 31, 32);
+      const int t = hl - base;
+      int lo = 0;  // the entry holding product t: the last e < ne with ex[e] <= t
+#pragma unroll
+      for (int step = 16; step > 0; step >>= 1) {
+        const int c = lo + step;
+        const int v = __shfl(ex, c < 32 ? c : 31, 32);
+        if (c < ne && v <= t) lo = c;
+      }
+      const RP st_e = __shfl(st, lo, 32);
+      const int ex_e = __shfl(ex, lo, 32);
+      const T av_e = __shfl(av, lo, 32);
+      if (t >= 0 && t < total) {  // the row has at most 32 products, so base + total <= 32
+        const RP q = st_e + (RP)(t - ex_e);
+        key = ((unsigned)tile.b_col[q] << kShift) | (unsigned)hl;
+        if (NUM) pv = (double)av_e * (double)tile.b_val[q];
+      }
+      base += total;
+    }
+    key = small_sort(key, hl);
+    // compress: a sorted position starts an output entry when its column differs from the previous position's
+    const unsigned col = key >> kShift;
+    const unsigned prev = __shfl_up(key, 1, 32);
+    const bool head = key != kEscPad && (hl == 0 || (prev >> kShift) != col);
+    const unsigned hmask = (unsigned)(__ballot(head) >> (32 * half));
+    const int cnt = __popc(hmask);
+    if (MODE == 0) {
+      if (active && hl == 0) R.nnz[fr] = cnt;
+      continue;
+    }
+    long long o = 0;
+    if (active && hl == 0) {
+      if (MODE == 1 || (MODE == 2 && cnt <= R.cap[fr])) {
+        o = R.off[fr];
+      } else {
+        o = (long long)atomicAdd(R.cursor, (unsigned long long)cnt);
+        R.off[fr] = o;
+      }
+      if (o + cnt > R.out_limit) {  // refused: nothing is written and, after this pass, the row holds 0 entries
+        R.stats[3] = 1ull;
+        o = -1;
+        if (MODE != 1) R.off[fr] = 0;
+      }
+      R.nnz[fr] = (o < 0 && MODE != 1) ? 0 : cnt;
+    }
+    o = __shfl(o, 0, 32);
+    // sum each run of equal columns in sorted order, i.e. in product order, by the run's first lane
+    const double mine = __shfl(pv, (int)(key & 31u), 32);
+    double sum = mine;
+    bool run = head;
+    for (int j = 1; j < 32; j++) {
+      const unsigned kj = __shfl_down(key, j, 32);
+      const double vj = __shfl_down(mine, j, 32);
+      run = run && hl + j < 32 && kj != kEscPad && (kj >> kShift) == col;
+      if (run) sum += vj;
+      if (__ballot(run) == 0ull) break;
+    }
+    if (head && o >= 0) {
+      const long long pos = o + __popc(hmask & ((1u << hl) - 1u));
+      tile.c_col[pos] = (int)col;
+      tile.c_val[pos] = (T)sum;
+    }
   }
 }
 

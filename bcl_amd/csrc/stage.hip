@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_bin_predicted(RowsDev<T, RP> R) {
     if (idx < cnt) {
       fr = R.pred_items[idx];
       const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
-      b = bin_for(R.cap[fr], R.prod[fr], R.tn, tile.n <= (1 << kEscColBits));
+      b = bin_for(R.cap[fr], R.prod[fr], R.tn, tile.n);
       pos = atomicAdd(&s_cnt[b], 1);
     }
     __syncthreads();
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_ovf_bin(RowsDev<T, RP> R) {
     const TileDev<T, RP> tile = R.tiles[R.blk_tile[fr >> 8]];
     const long long prod = R.prod[fr];
     const long long ub = prod < (long long)tile.n ? prod : (long long)tile.n;
-    const int b = bin_for(ub, prod, R.tn, tile.n <= (1 << kEscColBits));
+    const int b = bin_for(ub, prod, R.tn, tile.n);
     const int pos = atomicAdd(R.count_of(b), 1);
     R.bin_items[(int64_t)b * R.bin_stride + pos] = fr;
   }
@@ -252,6 +252,7 @@ struct StagePlan : StageBase {
       if (counts[b] == 0) continue;
       F.run_bin(side[q++ % side.size()], mode_k, b, counts[b]);
     }
+    if (counts[kSmallBin] > 0) F.run_bin(side[q++ % side.size()], mode_k, kSmallBin, counts[kSmallBin]);
     for (int b = 9; b >= 1; b--) {
       if (counts[b] == 0) continue;
       F.run_bin(side[q++ % side.size()], mode_k, b, counts[b]);
@@ -566,6 +567,7 @@ struct StagePlan : StageBase {
                    "stage output exceeded the capacity sized at plan time (inputs changed since the plan?)");
       for (auto& t : F.tiles) rows += t.m;
       for (int b = kEscBin0; b < kEscBin0 + kEscBins; b++) st[7] += counts1[b];
+      st[7] += counts1[kSmallBin];
     }
     st[5] = rows;
     for (int q = 0; q < n && q < 8; q++) out[q] = st[q];
