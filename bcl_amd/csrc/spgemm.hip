@@ -237,12 +237,16 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
     unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, div_up(count_hint, 2 * kSmallWaves))
                                     : (unsigned)(h->num_cus * 8);
     grid = std::min<unsigned>(grid, 1u << 20);
-    switch (mode) {
-      case 0: hipLaunchKernelGGL((k_esc_small<0, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b); break;
-      case 1: hipLaunchKernelGGL((k_esc_small<1, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b); break;
-      case 2: hipLaunchKernelGGL((k_esc_small<2, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b); break;
-      default: hipLaunchKernelGGL((k_esc_small<3, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b); break;
+#define BCLX_SMALL(M, ONE) \
+  hipLaunchKernelGGL((k_esc_small<M, ONE, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b)
+    if (ntiles == 1) {
+      switch (mode) { case 0: BCLX_SMALL(0, true); break; case 1: BCLX_SMALL(1, true); break;
+                      case 2: BCLX_SMALL(2, true); break; default: BCLX_SMALL(3, true); break; }
+    } else {
+      switch (mode) { case 0: BCLX_SMALL(0, false); break; case 1: BCLX_SMALL(1, false); break;
+                      case 2: BCLX_SMALL(2, false); break; default: BCLX_SMALL(3, false); break; }
     }
+#undef BCLX_SMALL
     BCLX_HIP(hipGetLastError());
     return;
   }
@@ -442,9 +446,11 @@ int64_t TwoPhase<T, RP>::symbolic() {
   hipStream_t s = h->stream;
   F.reset_counters(s);
   F.analyze(s, true);
-  int32_t counts[kNumBins];
+  int32_t* counts = sym_counts;
   F.read_counts(s, counts);
   if (counts[kHeavyBin] > 0) F.prepare_heavy();
+  reuse_bins = true;  // the hash bins and the heavy bin are chosen by the row's size bound, which the count replaces
+  for (int b = 1; b <= kHeavyBin; b++) reuse_bins = reuse_bins && counts[b] == 0;
   for (int b = 1; b < kNumBins; b++) F.run_bin(s, 0, b, counts[b]);
   F.scan(s, F.R.nnz, F.R.off, 0);
   int64_t tot = 0;
@@ -467,9 +473,13 @@ void TwoPhase<T, RP>::numeric(int* c_col, T* c_val, int64_t c_cap, void* row_ptr
     F.upload_tiles();
   }
   F.R.out_limit = c_cap;
-  F.bin(s, 0);
   int32_t counts[kNumBins];
-  F.read_counts(s, counts);
+  if (reuse_bins) {
+    std::copy(sym_counts, sym_counts + kNumBins, counts);
+  } else {
+    F.bin(s, 0);
+    F.read_counts(s, counts);
+  }
   for (int b = 1; b < kNumBins; b++) F.run_bin(s, 1, b, counts[b]);
   const unsigned g = div_up(m + 1, 256);
   if (ptr_width == 8)

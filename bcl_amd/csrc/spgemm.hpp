@@ -61,6 +61,10 @@ struct TwoPhase {
   FlatRows<T, RP> F;
   int64_t nnz_c = -1;
   int64_t m = 0;
+  // bins of the symbolic pass; when no row hashed, the numeric pass reuses them (sorted rows are binned by their
+  // products alone)
+  int32_t sym_counts[kNumBins] = {0};
+  bool reuse_bins = false;
   // scratch / scratch_bytes: device memory the workspace is laid out in instead of being allocated
   TwoPhase(bclx_handle* h_, const TileDev<T, RP>& tile, const BinTuning& tn, void* scratch = nullptr,
            size_t scratch_bytes = 0)

@@ -1047,111 +1047,150 @@ __device__ inline unsigned small_sort(unsigned key, int hl) {
   return key;
 }
 
-template <int MODE, typename T, typename RP>
-__global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R, int bin) {
+// One row per half wavefront: expand, sort, compress, write. `next` runs once the row's last global loads are
+// issued (the caller's prefetch of its next row overlaps them). Every exchange is executed by both halves.
+template <int MODE, typename T, typename RP, typename NEXT>
+__device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& tile, int fr, bool active, RP a0,
+                                 RP a1, int half, int hl, NEXT&& next) {
   constexpr bool NUM = MODE != 0;
   constexpr unsigned kShift = 32 - kSmallColBits;
+  const int na = (int)(a1 - a0);
+  const int na_max = max(na, __shfl_xor(na, 32, 64));
+  // expand: product x (A-row order) goes to lane x of the half
+  unsigned key = kEscPad;
+  double pv = 0.0;
+  int base = 0;
+  for (int c0 = 0; c0 < na_max; c0 += 32) {
+    const int ne = min(max(na - c0, 0), 32);
+    int len = 0;
+    RP st = 0;
+    T av = T(0);
+    if (hl < ne) {
+      const int kk = tile.a_col[a0 + c0 + hl];
+      st = tile.b_rp[kk];
+      len = (int)(tile.b_rp[kk + 1] - st);
+      if (NUM) av = tile.a_val[a0 + c0 + hl];
+    }
+    int inc = len;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+      const int t = __shfl_up(inc, d, 32);
+      if (hl >= d) inc += t;
+    }
+    const int ex = inc - len;
+    const int total = __shfl(inc, 31, 32);
+    const int t = hl - base;
+    int lo = 0;  // the entry holding product t: the last e < ne with ex[e] <= t
+#pragma unroll
+    for (int step = 16; step > 0; step >>= 1) {
+      const int c = lo + step;
+      const int v = __shfl(ex, c < 32 ? c : 31, 32);
+      if (c < ne && v <= t) lo = c;
+    }
+    const RP st_e = __shfl(st, lo, 32);
+    const int ex_e = __shfl(ex, lo, 32);
+    const T av_e = __shfl(av, lo, 32);
+    if (t >= 0 && t < total) {  // the row has at most 32 products, so base + total <= 32
+      const RP q = st_e + (RP)(t - ex_e);
+      key = ((unsigned)tile.b_col[q] << kShift) | (unsigned)hl;
+      if (NUM) pv = (double)av_e * (double)tile.b_val[q];
+    }
+    base += total;
+  }
+  next();
+  key = small_sort(key, hl);
+  // compress: a sorted position starts an output entry when its column differs from the previous position's
+  const unsigned col = key >> kShift;
+  const unsigned prev = __shfl_up(key, 1, 32);
+  const bool head = key != kEscPad && (hl == 0 || (prev >> kShift) != col);
+  const unsigned hmask = (unsigned)(__ballot(head) >> (32 * half));
+  const int cnt = __popc(hmask);
+  if (MODE == 0) {
+    if (active && hl == 0) R.nnz[fr] = cnt;
+    return;
+  }
+  long long o = 0;
+  if (active && hl == 0) {
+    if (MODE == 1 || (MODE == 2 && cnt <= R.cap[fr])) {
+      o = R.off[fr];
+    } else {
+      o = (long long)atomicAdd(R.cursor, (unsigned long long)cnt);
+      R.off[fr] = o;
+    }
+    if (o + cnt > R.out_limit) {  // refused: nothing is written and, after this pass, the row holds 0 entries
+      R.stats[3] = 1ull;
+      o = -1;
+      if (MODE != 1) R.off[fr] = 0;
+    }
+    R.nnz[fr] = (o < 0 && MODE != 1) ? 0 : cnt;
+  }
+  o = __shfl(o, 0, 32);
+  // sum each run of equal columns in sorted order, i.e. in product order, by the run's first lane
+  const double mine = __shfl(pv, (int)(key & 31u), 32);
+  double sum = mine;
+  bool run = head;
+  for (int j = 1; j < 32; j++) {
+    const unsigned kj = __shfl_down(key, j, 32);
+    const double vj = __shfl_down(mine, j, 32);
+    run = run && hl + j < 32 && kj != kEscPad && (kj >> kShift) == col;
+    if (run) sum += vj;
+    if (__ballot(run) == 0ull) break;
+  }
+  if (head && o >= 0) {
+    const long long pos = o + __popc(hmask & ((1u << hl) - 1u));
+    tile.c_col[pos] = (int)col;
+    tile.c_val[pos] = (T)sum;
+  }
+}
+
+// ONE: every row belongs to tiles[0] (a single product): the tile is read once, and the next row's bin item and
+// A-row bounds are loaded while the current row runs, so a row waits on three dependent loads (A columns, B-row
+// bounds, B entries) instead of five.
+template <int MODE, bool ONE, typename T, typename RP>
+__global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R, int bin) {
   const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
   const int count = *R.count_of(bin);
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
-  // two rows per wavefront; the loop and every exchange below are wave-uniform (both halves run them)
-  for (int p = (blockIdx.x * kSmallWaves + (threadIdx.x >> 6)) * 2; p < count; p += gridDim.x * kSmallWaves * 2) {
-    const int idx = p + half;
-    const bool active = idx < count;
-    const int fr = active ? items[idx] : 0;
-    TileDev<T, RP> tile;
+  const int stride = gridDim.x * kSmallWaves * 2;
+  int p = (blockIdx.x * kSmallWaves + (threadIdx.x >> 6)) * 2;  // two rows per wavefront; the loop is wave-uniform
+  if constexpr (ONE) {
+    const TileDev<T, RP> tile = R.tiles[0];
+    bool active = p + half < count;
+    int fr = active ? items[p + half] : 0;
     RP a0 = 0, a1 = 0;
     if (active) {
-      tile = R.tiles[R.blk_tile[fr >> 8]];
-      const int64_t i = fr - tile.flat0;
-      a0 = tile.a_rp[i];
-      a1 = tile.a_rp[i + 1];
+      a0 = tile.a_rp[fr - tile.flat0];
+      a1 = tile.a_rp[fr - tile.flat0 + 1];
     }
-    const int na = (int)(a1 - a0);
-    const int na_max = max(na, __shfl_xor(na, 32, 64));
-    // expand: product x (A-row order) goes to lane x of the half
-    unsigned key = kEscPad;
-    double pv = 0.0;
-    int base = 0;
-    for (int c0 = 0; c0 < na_max; c0 += 32) {
-      const int ne = min(max(na - c0, 0), 32);
-      int len = 0;
-      RP st = 0;
-      T av = T(0);
-      if (hl < ne) {
-        const int kk = tile.a_col[a0 + c0 + hl];
-        st = tile.b_rp[kk];
-        len = (int)(tile.b_rp[kk + 1] - st);
-        if (NUM) av = tile.a_val[a0 + c0 + hl];
-      }
-      int inc = len;
-#pragma unroll
-      for (int d = 1; d < 32; d <<= 1) {
-        const int t = __shfl_up(inc, d, 32);
-        if (hl >= d) inc += t;
-      }
-      const int ex = inc - len;
-      const int total = __shfl(inc, 31, 32);
-      const int t = hl - base;
-      int lo = 0;  // the entry holding product t: the last e < ne with ex[e] <= t
-#pragma unroll
-      for (int step = 16; step > 0; step >>= 1) {
-        const int c = lo + step;
-        const int v = __shfl(ex, c < 32 ? c : 31, 32);
-        if (c < ne && v <= t) lo = c;
-      }
-      const RP st_e = __shfl(st, lo, 32);
-      const int ex_e = __shfl(ex, lo, 32);
-      const T av_e = __shfl(av, lo, 32);
-      if (t >= 0 && t < total) {  // the row has at most 32 products, so base + total <= 32
-        const RP q = st_e + (RP)(t - ex_e);
-        key = ((unsigned)tile.b_col[q] << kShift) | (unsigned)hl;
-        if (NUM) pv = (double)av_e * (double)tile.b_val[q];
-      }
-      base += total;
+    for (; p < count; p += stride) {
+      const bool act_n = p + stride + half < count;
+      const int fr_n = act_n ? items[p + stride + half] : 0;
+      RP a0_n = 0, a1_n = 0;
+      small_row<MODE>(R, tile, fr, active, a0, a1, half, hl, [&] {
+        if (act_n) {
+          a0_n = tile.a_rp[fr_n - tile.flat0];
+          a1_n = tile.a_rp[fr_n - tile.flat0 + 1];
+        }
+      });
+      active = act_n;
+      fr = fr_n;
+      a0 = a0_n;
+      a1 = a1_n;
     }
-    key = small_sort(key, hl);
-    // compress: a sorted position starts an output entry when its column differs from the previous position's
-    const unsigned col = key >> kShift;
-    const unsigned prev = __shfl_up(key, 1, 32);
-    const bool head = key != kEscPad && (hl == 0 || (prev >> kShift) != col);
-    const unsigned hmask = (unsigned)(__ballot(head) >> (32 * half));
-    const int cnt = __popc(hmask);
-    if (MODE == 0) {
-      if (active && hl == 0) R.nnz[fr] = cnt;
-      continue;
-    }
-    long long o = 0;
-    if (active && hl == 0) {
-      if (MODE == 1 || (MODE == 2 && cnt <= R.cap[fr])) {
-        o = R.off[fr];
-      } else {
-        o = (long long)atomicAdd(R.cursor, (unsigned long long)cnt);
-        R.off[fr] = o;
+  } else {
+    for (; p < count; p += stride) {
+      const int idx = p + half;
+      const bool active = idx < count;
+      const int fr = active ? items[idx] : 0;
+      TileDev<T, RP> tile;
+      RP a0 = 0, a1 = 0;
+      if (active) {
+        tile = R.tiles[R.blk_tile[fr >> 8]];
+        const int64_t i = fr - tile.flat0;
+        a0 = tile.a_rp[i];
+        a1 = tile.a_rp[i + 1];
       }
-      if (o + cnt > R.out_limit) {  // refused: nothing is written and, after this pass, the row holds 0 entries
-        R.stats[3] = 1ull;
-        o = -1;
-        if (MODE != 1) R.off[fr] = 0;
-      }
-      R.nnz[fr] = (o < 0 && MODE != 1) ? 0 : cnt;
-    }
-    o = __shfl(o, 0, 32);
-    // sum each run of equal columns in sorted order, i.e. in product order, by the run's first lane
-    const double mine = __shfl(pv, (int)(key & 31u), 32);
-    double sum = mine;
-    bool run = head;
-    for (int j = 1; j < 32; j++) {
-      const unsigned kj = __shfl_down(key, j, 32);
-      const double vj = __shfl_down(mine, j, 32);
-      run = run && hl + j < 32 && kj != kEscPad && (kj >> kShift) == col;
-      if (run) sum += vj;
-      if (__ballot(run) == 0ull) break;
-    }
-    if (head && o >= 0) {
-      const long long pos = o + __popc(hmask & ((1u << hl) - 1u));
-      tile.c_col[pos] = (int)col;
-      tile.c_val[pos] = (T)sum;
+      small_row<MODE>(R, tile, fr, active, a0, a1, half, hl, [] {});
     }
   }
 }
