@@ -1605,11 +1605,24 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
       long long total;
       long long run = block_excl_scan<BLOCK, long long>(c, &total, s_scan);
       advance(3);
+      const int run0 = (int)run;
 #pragma unroll
       for (int q = 0; q < kGroupsPerThread; q++) {
         const int g = tid * kGroupsPerThread + q;
         if (NUM && g < ng) s_gpre[g] = (int)run;
         run += gc[q];
+      }
+      // A row whose columns fit one slice (most heavy rows) stages them here, by rank, from the words this thread
+      // just counted: no separate pass over the bitmap and no column search.
+      if (NUM && total <= (long long)kHRSlice) {
+        int r = run0;
+        for (int w = tid * kGroupsPerThread * kHRGroup; w < (tid + 1) * kGroupsPerThread * kHRGroup && w < Wr; w++) {
+          unsigned m = s_bits[w];
+          while (m) {
+            s_colbuf[r++] = w * 32 + (__ffs(m) - 1);
+            m &= m - 1;
+          }
+        }
       }
       if (MODE == 0) {
         if (tid == 0) {
@@ -1648,16 +1661,21 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
       RP my_lo = 0;
       for (long long s0 = 0; s0 < total; s0 += kHRSlice) {
         const int len = (int)(total - s0 < kHRSlice ? total - s0 : kHRSlice);
-        const bool whole = s0 == 0 && len == total;
-        if (tid == 0) {  // column range of the slice
-          s_crange[0] = s0 == 0 ? 0 : s_crange[1];
-          s_crange[1] = s0 + len >= total ? INT_MAX : hr_select(s_bits, s_gpre, ng, (int)(s0 + len));
+        const bool whole = s0 == 0 && len == total;  // the columns are staged already (above)
+        int c_lo = 0, c_hi = INT_MAX;
+        if (!whole) {
+          if (tid == 0) {  // column range of the slice
+            s_crange[0] = s0 == 0 ? 0 : s_crange[1];
+            s_crange[1] = s0 + len >= total ? INT_MAX : hr_select(s_bits, s_gpre, ng, (int)(s0 + len));
+          }
+          __syncthreads();
+          c_lo = s_crange[0];
+          c_hi = s_crange[1];
         }
-        __syncthreads();
-        const int c_lo = s_crange[0], c_hi = s_crange[1];
         const int w_lo = c_lo >> 5, w_hi = c_hi == INT_MAX ? W - 1 : (c_hi - 1) >> 5;
         // a wavefront takes 64 consecutive words (whole groups) at a time: each word's rank base is its group prefix
         // plus the bits of the group's earlier words (a kHRGroup-lane scan of popcounts)
+        if (!whole)
         for (int wb = (w_lo & ~63) + (tid & ~63); wb <= w_hi; wb += BLOCK) {
           const int lane = tid & 63, w = wb + lane;
           const unsigned full = w < W ? s_bits[w] : 0u;
