@@ -234,11 +234,11 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
   // count_hint >= 0: exact row count known on the host (grid sized to it); < 0: persistent grid.
   if (b == 0 || count_hint == 0) return;
   if (b == kSmallBin) {
-    unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, div_up(count_hint, 2 * kSmallWaves))
+    unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, div_up(count_hint, (64 / kSmallGroup) * kSmallWaves))
                                     : (unsigned)(h->num_cus * 8);
     grid = std::min<unsigned>(grid, 1u << 20);
 #define BCLX_SMALL(M, ONE) \
-  hipLaunchKernelGGL((k_esc_small<M, ONE, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b)
+  hipLaunchKernelGGL((k_esc_small<M, ONE, kSmallGroup, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b)
     if (ntiles == 1) {
       switch (mode) { case 0: BCLX_SMALL(0, true); break; case 1: BCLX_SMALL(1, true); break;
                       case 2: BCLX_SMALL(2, true); break; default: BCLX_SMALL(3, true); break; }

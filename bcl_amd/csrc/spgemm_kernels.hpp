@@ -27,6 +27,7 @@ constexpr int kEscColBits = 19;                         // sort keys pack column
 constexpr int kSmallBin = 18;
 constexpr int kSmallMax = 32;       // products per row in the small bin
 constexpr int kSmallColBits = 27;   // small-bin keys pack column (< 2^27 - 1) and product index (5 bits)
+constexpr int kSmallGroup = 16;     // lanes per small-bin row (two products per lane)
 constexpr int kRowBlock = 256;         // flat-row padding and analysis block size
 constexpr int kHeavyThreads = 1024;
 
@@ -1020,97 +1021,146 @@ __global__ __launch_bounds__(64 * kEscWaves) void k_esc(RowsDev<T, RP> R, int bi
 
 constexpr int kSmallWaves = 4;  // wavefronts per block
 
-template <int K, int J>
-__device__ inline unsigned small_cx(unsigned key, int hl) {
-  const unsigned other = lane_xor<J>(key, hl);
-  const bool lower = (hl & J) == 0, asc = (hl & K) == 0;
-  return lower == asc ? min(key, other) : max(key, other);
+// One bitonic stage (block size K, partner distance J) over the 32 elements of a row group: element x = s * G + gl
+// sits in slot s of lane gl. Partners J >= G lie in the same lane (another slot); closer ones in lane gl ^ J.
+template <int K, int J, int G, int S>
+__device__ inline void grp_cx(unsigned (&key)[S], int gl) {
+  if constexpr (J >= G) {
+    constexpr int D = J / G;
+#pragma unroll
+    for (int s = 0; s < S; s++)
+      if ((s & D) == 0) {
+        const bool asc = ((s * G + gl) & K) == 0;
+        const unsigned a = key[s], b = key[s + D];
+        key[s] = asc ? min(a, b) : max(a, b);
+        key[s + D] = asc ? max(a, b) : min(a, b);
+      }
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const unsigned other = lane_xor<J>(key[s], gl);
+      const int x = s * G + gl;
+      const bool lower = (x & J) == 0, asc = (x & K) == 0;
+      key[s] = lower == asc ? min(key[s], other) : max(key[s], other);
+    }
+  }
 }
 
-// ascending over the 32 lanes of each half wavefront
-__device__ inline unsigned small_sort(unsigned key, int hl) {
-  key = small_cx<2, 1>(key, hl);
-  key = small_cx<4, 2>(key, hl);
-  key = small_cx<4, 1>(key, hl);
-  key = small_cx<8, 4>(key, hl);
-  key = small_cx<8, 2>(key, hl);
-  key = small_cx<8, 1>(key, hl);
-  key = small_cx<16, 8>(key, hl);
-  key = small_cx<16, 4>(key, hl);
-  key = small_cx<16, 2>(key, hl);
-  key = small_cx<16, 1>(key, hl);
-  key = small_cx<32, 16>(key, hl);
-  key = small_cx<32, 8>(key, hl);
-  key = small_cx<32, 4>(key, hl);
-  key = small_cx<32, 2>(key, hl);
-  key = small_cx<32, 1>(key, hl);
-  return key;
+// ascending over the 32 elements of each row group
+template <int G, int S>
+__device__ inline void grp_sort(unsigned (&key)[S], int gl) {
+  grp_cx<2, 1, G, S>(key, gl);
+  grp_cx<4, 2, G, S>(key, gl);
+  grp_cx<4, 1, G, S>(key, gl);
+  grp_cx<8, 4, G, S>(key, gl);
+  grp_cx<8, 2, G, S>(key, gl);
+  grp_cx<8, 1, G, S>(key, gl);
+  grp_cx<16, 8, G, S>(key, gl);
+  grp_cx<16, 4, G, S>(key, gl);
+  grp_cx<16, 2, G, S>(key, gl);
+  grp_cx<16, 1, G, S>(key, gl);
+  grp_cx<32, 16, G, S>(key, gl);
+  grp_cx<32, 8, G, S>(key, gl);
+  grp_cx<32, 4, G, S>(key, gl);
+  grp_cx<32, 2, G, S>(key, gl);
+  grp_cx<32, 1, G, S>(key, gl);
 }
 
-// One row per half wavefront: expand, sort, compress, write. `next` runs once the row's last global loads are
-// issued (the caller's prefetch of its next row overlaps them). Every exchange is executed by both halves.
-template <int MODE, typename T, typename RP, typename NEXT>
+// The value of slot-array v at element x of the group (lane x % G, slot x / G). Every lane runs the exchanges.
+template <int G, int S, typename V>
+__device__ inline V grp_at(const V (&v)[S], int x) {
+  V r = V(0);
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const V t = __shfl(v[s], x & (G - 1), G);
+    if (x / G == s) r = t;
+  }
+  return r;
+}
+
+// One row per group of G lanes (G = 32: half a wavefront; G = 16: a quarter, two products per lane): expand, sort,
+// compress, write. `next` runs once the row's last global loads are issued (the caller's prefetch of its next row
+// overlaps them). Every exchange is executed by all groups of the wavefront.
+template <int MODE, int G, typename T, typename RP, typename NEXT>
 __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& tile, int fr, bool active, RP a0,
-                                 RP a1, int half, int hl, NEXT&& next) {
+                                 RP a1, int grp, int gl, NEXT&& next) {
   constexpr bool NUM = MODE != 0;
+  constexpr int S = 32 / G;
   constexpr unsigned kShift = 32 - kSmallColBits;
   const int na = (int)(a1 - a0);
-  const int na_max = max(na, __shfl_xor(na, 32, 64));
-  // expand: product x (A-row order) goes to lane x of the half
-  unsigned key = kEscPad;
-  double pv = 0.0;
+  int na_max = na;
+#pragma unroll
+  for (int d = G; d < 64; d <<= 1) na_max = max(na_max, __shfl_xor(na_max, d, 64));
+  // expand: product x (A-row order) goes to slot x / G of lane x % G of the group
+  unsigned key[S];
+  double pv[S];
+#pragma unroll
+  for (int q = 0; q < S; q++) {
+    key[q] = kEscPad;
+    pv[q] = 0.0;
+  }
   int base = 0;
-  for (int c0 = 0; c0 < na_max; c0 += 32) {
-    const int ne = min(max(na - c0, 0), 32);
+  for (int c0 = 0; c0 < na_max; c0 += G) {
+    const int ne = min(max(na - c0, 0), G);
     int len = 0;
     RP st = 0;
     T av = T(0);
-    if (hl < ne) {
-      const int kk = tile.a_col[a0 + c0 + hl];
+    if (gl < ne) {
+      const int kk = tile.a_col[a0 + c0 + gl];
       st = tile.b_rp[kk];
       len = (int)(tile.b_rp[kk + 1] - st);
-      if (NUM) av = tile.a_val[a0 + c0 + hl];
+      if (NUM) av = tile.a_val[a0 + c0 + gl];
     }
     int inc = len;
 #pragma unroll
-    for (int d = 1; d < 32; d <<= 1) {
-      const int t = __shfl_up(inc, d, 32);
-      if (hl >= d) inc += t;
+    for (int d = 1; d < G; d <<= 1) {
+      const int t = __shfl_up(inc, d, G);
+      if (gl >= d) inc += t;
     }
     const int ex = inc - len;
-    const int total = __shfl(inc, 31, 32);
-    const int t = hl - base;
-    int lo = 0;  // the entry holding product t: the last e < ne with ex[e] <= t
+    const int total = __shfl(inc, G - 1, G);
 #pragma unroll
-    for (int step = 16; step > 0; step >>= 1) {
-      const int c = lo + step;
-      const int v = __shfl(ex, c < 32 ? c : 31, 32);
-      if (c < ne && v <= t) lo = c;
-    }
-    const RP st_e = __shfl(st, lo, 32);
-    const int ex_e = __shfl(ex, lo, 32);
-    const T av_e = __shfl(av, lo, 32);
-    if (t >= 0 && t < total) {  // the row has at most 32 products, so base + total <= 32
-      const RP q = st_e + (RP)(t - ex_e);
-      key = ((unsigned)tile.b_col[q] << kShift) | (unsigned)hl;
-      if (NUM) pv = (double)av_e * (double)tile.b_val[q];
+    for (int q = 0; q < S; q++) {
+      const int t = gl + G * q - base;
+      int lo = 0;  // the entry holding product t: the last e < ne with ex[e] <= t
+#pragma unroll
+      for (int step = G / 2; step > 0; step >>= 1) {
+        const int c = lo + step;
+        const int v = __shfl(ex, c < G ? c : G - 1, G);
+        if (c < ne && v <= t) lo = c;
+      }
+      const RP st_e = __shfl(st, lo, G);
+      const int ex_e = __shfl(ex, lo, G);
+      const T av_e = __shfl(av, lo, G);
+      if (t >= 0 && t < total) {  // the row has at most 32 products, so base + total <= 32
+        const RP qb = st_e + (RP)(t - ex_e);
+        key[q] = ((unsigned)tile.b_col[qb] << kShift) | (unsigned)(gl + G * q);
+        if (NUM) pv[q] = (double)av_e * (double)tile.b_val[qb];
+      }
     }
     base += total;
   }
   next();
-  key = small_sort(key, hl);
+  grp_sort<G, S>(key, gl);
   // compress: a sorted position starts an output entry when its column differs from the previous position's
-  const unsigned col = key >> kShift;
-  const unsigned prev = __shfl_up(key, 1, 32);
-  const bool head = key != kEscPad && (hl == 0 || (prev >> kShift) != col);
-  const unsigned hmask = (unsigned)(__ballot(head) >> (32 * half));
-  const int cnt = __popc(hmask);
+  unsigned hm[S];
+  bool head[S];
+  int cnt = 0;
+#pragma unroll
+  for (int q = 0; q < S; q++) {
+    const unsigned up = __shfl_up(key[q], 1, G);
+    const unsigned wrap = q > 0 ? __shfl(key[q > 0 ? q - 1 : 0], G - 1, G) : 0u;
+    const unsigned prev = gl > 0 ? up : wrap;
+    head[q] = key[q] != kEscPad && (gl + q * G == 0 || (prev >> kShift) != (key[q] >> kShift));
+    hm[q] = (unsigned)((__ballot(head[q]) >> (grp * G)) & (G == 32 ? 0xffffffffull : ((1ull << G) - 1)));
+    cnt += __popc(hm[q]);
+  }
   if (MODE == 0) {
-    if (active && hl == 0) R.nnz[fr] = cnt;
+    if (active && gl == 0) R.nnz[fr] = cnt;
     return;
   }
   long long o = 0;
-  if (active && hl == 0) {
+  if (active && gl == 0) {
     if (MODE == 1 || (MODE == 2 && cnt <= R.cap[fr])) {
       o = R.off[fr];
     } else {
@@ -1124,49 +1174,75 @@ __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& 
     }
     R.nnz[fr] = (o < 0 && MODE != 1) ? 0 : cnt;
   }
-  o = __shfl(o, 0, 32);
-  // sum each run of equal columns in sorted order, i.e. in product order, by the run's first lane
-  const double mine = __shfl(pv, (int)(key & 31u), 32);
-  double sum = mine;
-  bool run = head;
-  for (int j = 1; j < 32; j++) {
-    const unsigned kj = __shfl_down(key, j, 32);
-    const double vj = __shfl_down(mine, j, 32);
-    run = run && hl + j < 32 && kj != kEscPad && (kj >> kShift) == col;
-    if (run) sum += vj;
-    if (__ballot(run) == 0ull) break;
+  o = __shfl(o, 0, G);
+  // sum each run of equal columns in sorted order, i.e. in product order, by the run's first element
+  double mine[S], sum[S];
+  bool run[S];
+#pragma unroll
+  for (int q = 0; q < S; q++) {
+    const int pidx = (int)(key[q] & 31u);
+    double got = 0.0;
+#pragma unroll
+    for (int r = 0; r < S; r++) {
+      const double t = __shfl(pv[r], pidx & (G - 1), G);
+      if (pidx / G == r) got = t;
+    }
+    mine[q] = got;
+    sum[q] = got;
+    run[q] = head[q];
   }
-  if (head && o >= 0) {
-    const long long pos = o + __popc(hmask & ((1u << hl) - 1u));
-    tile.c_col[pos] = (int)col;
-    tile.c_val[pos] = (T)sum;
+  for (int j = 1; j < 32; j++) {
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < S; q++) {
+      const int x = q * G + gl + j;  // the element j places after this one
+      const unsigned kj = grp_at<G, S>(key, x);
+      const double vj = grp_at<G, S>(mine, x);
+      run[q] = run[q] && x < 32 && kj != kEscPad && (kj >> kShift) == (key[q] >> kShift);
+      if (run[q]) sum[q] += vj;
+      any = any || run[q];
+    }
+    if (__ballot(any) == 0ull) break;
+  }
+  if (o >= 0) {
+    int before = 0;
+#pragma unroll
+    for (int q = 0; q < S; q++) {
+      if (head[q]) {
+        const long long pos = o + before + __popc(hm[q] & ((1u << gl) - 1u));
+        tile.c_col[pos] = (int)(key[q] >> kShift);
+        tile.c_val[pos] = (T)sum[q];
+      }
+      before += __popc(hm[q]);
+    }
   }
 }
 
 // ONE: every row belongs to tiles[0] (a single product): the tile is read once, and the next row's bin item and
 // A-row bounds are loaded while the current row runs, so a row waits on three dependent loads (A columns, B-row
-// bounds, B entries) instead of five.
-template <int MODE, bool ONE, typename T, typename RP>
+// bounds, B entries) instead of five. G: lanes per row (32 or 16).
+template <int MODE, bool ONE, int G, typename T, typename RP>
 __global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R, int bin) {
-  const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
+  constexpr int RW = 64 / G;  // rows per wavefront
+  const int lane = threadIdx.x & 63, grp = lane / G, gl = lane % G;
   const int count = *R.count_of(bin);
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
-  const int stride = gridDim.x * kSmallWaves * 2;
-  int p = (blockIdx.x * kSmallWaves + (threadIdx.x >> 6)) * 2;  // two rows per wavefront; the loop is wave-uniform
+  const int stride = gridDim.x * kSmallWaves * RW;
+  int p = (blockIdx.x * kSmallWaves + (threadIdx.x >> 6)) * RW;  // the loop is wave-uniform
   if constexpr (ONE) {
     const TileDev<T, RP> tile = R.tiles[0];
-    bool active = p + half < count;
-    int fr = active ? items[p + half] : 0;
+    bool active = p + grp < count;
+    int fr = active ? items[p + grp] : 0;
     RP a0 = 0, a1 = 0;
     if (active) {
       a0 = tile.a_rp[fr - tile.flat0];
       a1 = tile.a_rp[fr - tile.flat0 + 1];
     }
     for (; p < count; p += stride) {
-      const bool act_n = p + stride + half < count;
-      const int fr_n = act_n ? items[p + stride + half] : 0;
+      const bool act_n = p + stride + grp < count;
+      const int fr_n = act_n ? items[p + stride + grp] : 0;
       RP a0_n = 0, a1_n = 0;
-      small_row<MODE>(R, tile, fr, active, a0, a1, half, hl, [&] {
+      small_row<MODE, G>(R, tile, fr, active, a0, a1, grp, gl, [&] {
         if (act_n) {
           a0_n = tile.a_rp[fr_n - tile.flat0];
           a1_n = tile.a_rp[fr_n - tile.flat0 + 1];
@@ -1179,7 +1255,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R
     }
   } else {
     for (; p < count; p += stride) {
-      const int idx = p + half;
+      const int idx = p + grp;
       const bool active = idx < count;
       const int fr = active ? items[idx] : 0;
       TileDev<T, RP> tile;
@@ -1190,7 +1266,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R
         a0 = tile.a_rp[i];
         a1 = tile.a_rp[i + 1];
       }
-      small_row<MODE>(R, tile, fr, active, a0, a1, half, hl, [] {});
+      small_row<MODE, G>(R, tile, fr, active, a0, a1, grp, gl, [] {});
     }
   }
 }
