@@ -27,7 +27,7 @@ constexpr int kEscColBits = 19;                         // sort keys pack column
 constexpr int kSmallBin = 18;
 constexpr int kSmallMax = 32;       // products per row in the small bin
 constexpr int kSmallColBits = 27;   // small-bin keys pack column (< 2^27 - 1) and product index (5 bits)
-constexpr int kSmallGroup = 16;     // lanes per small-bin row (two products per lane)
+constexpr int kSmallGroup = 8;      // lanes per small-bin row (four products per lane; 16 and 4 measured slower)
 constexpr int kRowBlock = 256;         // flat-row padding and analysis block size
 constexpr int kHeavyThreads = 1024;
 
@@ -1078,7 +1078,8 @@ __device__ inline V grp_at(const V (&v)[S], int x) {
   return r;
 }
 
-// One row per group of G lanes (G = 32: half a wavefront; G = 16: a quarter, two products per lane): expand, sort,
+// One row per group of G lanes, 32 / G products per lane (G = 8: eight rows per wavefront; measured on the FP64
+// Laplacian square, 16.7 M rows of 25 products: G = 32 9.4 ms, 16 7.5 ms, 8 6.6 ms, 4 8.5 ms): expand, sort,
 // compress, write. `next` runs once the row's last global loads are issued (the caller's prefetch of its next row
 // overlaps them). Every exchange is executed by all groups of the wavefront.
 template <int MODE, int G, typename T, typename RP, typename NEXT>
