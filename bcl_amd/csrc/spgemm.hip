@@ -233,12 +233,14 @@ template <typename T, typename RP>
 void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint) {
   // count_hint >= 0: exact row count known on the host (grid sized to it); < 0: persistent grid.
   if (b == 0 || count_hint == 0) return;
-  if (b == kSmallBin) {
-    unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, div_up(count_hint, (64 / kSmallGroup) * kSmallWaves))
+  // rows of at most 32 products (bin 18) and of 33-64 (ESC bin 11): sorted in registers by groups of G lanes
+  auto launch_small = [&](auto g_tag, auto n_tag) {
+    constexpr int G = decltype(g_tag)::value, N = decltype(n_tag)::value;
+    unsigned grid = count_hint >= 0 ? (unsigned)std::max<int64_t>(1, div_up(count_hint, (64 / G) * kSmallWaves))
                                     : (unsigned)(h->num_cus * 8);
     grid = std::min<unsigned>(grid, 1u << 20);
 #define BCLX_SMALL(M, ONE) \
-  hipLaunchKernelGGL((k_esc_small<M, ONE, kSmallGroup, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b)
+  hipLaunchKernelGGL((k_esc_small<M, ONE, G, N, T, RP>), dim3(grid), dim3(64 * kSmallWaves), 0, s, R, b)
     if (ntiles == 1) {
       switch (mode) { case 0: BCLX_SMALL(0, true); break; case 1: BCLX_SMALL(1, true); break;
                       case 2: BCLX_SMALL(2, true); break; default: BCLX_SMALL(3, true); break; }
@@ -248,6 +250,13 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
     }
 #undef BCLX_SMALL
     BCLX_HIP(hipGetLastError());
+  };
+  if (b == kSmallBin) {
+    launch_small(std::integral_constant<int, kSmallGroup>{}, std::integral_constant<int, kSmallMax>{});
+    return;
+  }
+  if (b == kEscBin0) {
+    launch_small(std::integral_constant<int, 16>{}, std::integral_constant<int, 64>{});
     return;
   }
   if (b >= kEscBin0 && b < kEscBin0 + kEscWaveBins) {
@@ -255,7 +264,6 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
                                     : (unsigned)(h->num_cus * 8);
     grid = std::min<unsigned>(grid, 1u << 20);
     switch (b - kEscBin0) {
-      case 0: launch_esc<1>(R, s, mode, b, grid); break;
       case 1: launch_esc<2>(R, s, mode, b, grid); break;
       case 2: launch_esc<4>(R, s, mode, b, grid); break;
       case 3: launch_esc<8>(R, s, mode, b, grid); break;

@@ -1046,24 +1046,16 @@ __device__ inline void grp_cx(unsigned (&key)[S], int gl) {
   }
 }
 
-// ascending over the 32 elements of each row group
-template <int G, int S>
+// ascending over the N elements of each row group (bitonic: block sizes K = 2 .. N, partner distances J = K/2 .. 1)
+template <int K, int J, int G, int S, int N>
+__device__ inline void grp_sort_from(unsigned (&key)[S], int gl) {
+  grp_cx<K, J, G, S>(key, gl);
+  if constexpr (J > 1) grp_sort_from<K, J / 2, G, S, N>(key, gl);
+  else if constexpr (K < N) grp_sort_from<K * 2, K, G, S, N>(key, gl);
+}
+template <int G, int S, int N>
 __device__ inline void grp_sort(unsigned (&key)[S], int gl) {
-  grp_cx<2, 1, G, S>(key, gl);
-  grp_cx<4, 2, G, S>(key, gl);
-  grp_cx<4, 1, G, S>(key, gl);
-  grp_cx<8, 4, G, S>(key, gl);
-  grp_cx<8, 2, G, S>(key, gl);
-  grp_cx<8, 1, G, S>(key, gl);
-  grp_cx<16, 8, G, S>(key, gl);
-  grp_cx<16, 4, G, S>(key, gl);
-  grp_cx<16, 2, G, S>(key, gl);
-  grp_cx<16, 1, G, S>(key, gl);
-  grp_cx<32, 16, G, S>(key, gl);
-  grp_cx<32, 8, G, S>(key, gl);
-  grp_cx<32, 4, G, S>(key, gl);
-  grp_cx<32, 2, G, S>(key, gl);
-  grp_cx<32, 1, G, S>(key, gl);
+  grp_sort_from<2, 1, G, S, N>(key, gl);
 }
 
 // The value of slot-array v at element x of the group (lane x % G, slot x / G). Every lane runs the exchanges.
@@ -1082,12 +1074,13 @@ __device__ inline V grp_at(const V (&v)[S], int x) {
 // Laplacian square, 16.7 M rows of 25 products: G = 32 9.4 ms, 16 7.5 ms, 8 6.6 ms, 4 8.5 ms): expand, sort,
 // compress, write. `next` runs once the row's last global loads are issued (the caller's prefetch of its next row
 // overlaps them). Every exchange is executed by all groups of the wavefront.
-template <int MODE, int G, typename T, typename RP, typename NEXT>
+template <int MODE, int G, int N, typename T, typename RP, typename NEXT>
 __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& tile, int fr, bool active, RP a0,
                                  RP a1, int grp, int gl, NEXT&& next) {
   constexpr bool NUM = MODE != 0;
-  constexpr int S = 32 / G;
-  constexpr unsigned kShift = 32 - kSmallColBits;
+  constexpr int S = N / G;
+  constexpr unsigned kShift = N == 32 ? 5 : 6;  // product-index bits of a key
+  static_assert(N == 32 || N == 64, "rows of at most 32 or 64 products");
   const int na = (int)(a1 - a0);
   int na_max = na;
 #pragma unroll
@@ -1133,7 +1126,7 @@ __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& 
       const RP st_e = __shfl(st, lo, G);
       const int ex_e = __shfl(ex, lo, G);
       const T av_e = __shfl(av, lo, G);
-      if (t >= 0 && t < total) {  // the row has at most 32 products, so base + total <= 32
+      if (t >= 0 && t < total) {  // the row has at most N products, so base + total <= N
         const RP qb = st_e + (RP)(t - ex_e);
         key[q] = ((unsigned)tile.b_col[qb] << kShift) | (unsigned)(gl + G * q);
         if (NUM) pv[q] = (double)av_e * (double)tile.b_val[qb];
@@ -1142,7 +1135,7 @@ __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& 
     base += total;
   }
   next();
-  grp_sort<G, S>(key, gl);
+  grp_sort<G, S, N>(key, gl);
   // compress: a sorted position starts an output entry when its column differs from the previous position's
   unsigned hm[S];
   bool head[S];
@@ -1181,7 +1174,7 @@ __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& 
   bool run[S];
 #pragma unroll
   for (int q = 0; q < S; q++) {
-    const int pidx = (int)(key[q] & 31u);
+    const int pidx = (int)(key[q] & (unsigned)(N - 1));
     double got = 0.0;
 #pragma unroll
     for (int r = 0; r < S; r++) {
@@ -1192,14 +1185,14 @@ __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& 
     sum[q] = got;
     run[q] = head[q];
   }
-  for (int j = 1; j < 32; j++) {
+  for (int j = 1; j < N; j++) {
     bool any = false;
 #pragma unroll
     for (int q = 0; q < S; q++) {
       const int x = q * G + gl + j;  // the element j places after this one
       const unsigned kj = grp_at<G, S>(key, x);
       const double vj = grp_at<G, S>(mine, x);
-      run[q] = run[q] && x < 32 && kj != kEscPad && (kj >> kShift) == (key[q] >> kShift);
+      run[q] = run[q] && x < N && kj != kEscPad && (kj >> kShift) == (key[q] >> kShift);
       if (run[q]) sum[q] += vj;
       any = any || run[q];
     }
@@ -1222,7 +1215,7 @@ __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& 
 // ONE: every row belongs to tiles[0] (a single product): the tile is read once, and the next row's bin item and
 // A-row bounds are loaded while the current row runs, so a row waits on three dependent loads (A columns, B-row
 // bounds, B entries) instead of five. G: lanes per row (32 or 16).
-template <int MODE, bool ONE, int G, typename T, typename RP>
+template <int MODE, bool ONE, int G, int N, typename T, typename RP>
 __global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R, int bin) {
   constexpr int RW = 64 / G;  // rows per wavefront
   const int lane = threadIdx.x & 63, grp = lane / G, gl = lane % G;
@@ -1243,7 +1236,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R
       const bool act_n = p + stride + grp < count;
       const int fr_n = act_n ? items[p + stride + grp] : 0;
       RP a0_n = 0, a1_n = 0;
-      small_row<MODE, G>(R, tile, fr, active, a0, a1, grp, gl, [&] {
+      small_row<MODE, G, N>(R, tile, fr, active, a0, a1, grp, gl, [&] {
         if (act_n) {
           a0_n = tile.a_rp[fr_n - tile.flat0];
           a1_n = tile.a_rp[fr_n - tile.flat0 + 1];
@@ -1267,7 +1260,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R
         a0 = tile.a_rp[i];
         a1 = tile.a_rp[i + 1];
       }
-      small_row<MODE, G>(R, tile, fr, active, a0, a1, grp, gl, [] {});
+      small_row<MODE, G, N>(R, tile, fr, active, a0, a1, grp, gl, [] {});
     }
   }
 }
