@@ -1222,7 +1222,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R
   const int count = *R.count_of(bin);
   const int* items = R.bin_items + (int64_t)bin * R.bin_stride;
   const int stride = gridDim.x * kSmallWaves * RW;
-  int p = (blockIdx.x * kSmallWaves + (threadIdx.x >> 6)) * RW;  // the loop is wave-uniform
+  int64_t p = (int64_t)(blockIdx.x * kSmallWaves + (threadIdx.x >> 6)) * RW;  // the loop is wave-uniform
   if constexpr (ONE) {
     const TileDev<T, RP> tile = R.tiles[0];
     bool active = p + grp < count;
@@ -1233,8 +1233,9 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R
       a1 = tile.a_rp[fr - tile.flat0 + 1];
     }
     for (; p < count; p += stride) {
-      const bool act_n = p + stride + grp < count;
-      const int fr_n = act_n ? items[p + stride + grp] : 0;
+      const int64_t idx_n = p + stride + grp;
+      const bool act_n = idx_n < count;
+      const int fr_n = act_n ? items[idx_n] : 0;
       RP a0_n = 0, a1_n = 0;
       small_row<MODE, G, N>(R, tile, fr, active, a0, a1, grp, gl, [&] {
         if (act_n) {
@@ -1249,7 +1250,7 @@ __global__ __launch_bounds__(64 * kSmallWaves) void k_esc_small(RowsDev<T, RP> R
     }
   } else {
     for (; p < count; p += stride) {
-      const int idx = p + grp;
+      const int64_t idx = p + grp;
       const bool active = idx < count;
       const int fr = active ? items[idx] : 0;
       TileDev<T, RP> tile;
