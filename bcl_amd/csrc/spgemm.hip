@@ -266,8 +266,7 @@ void FlatRows<T, RP>::run_bin(hipStream_t s, int mode, int b, int64_t count_hint
     switch (b - kEscBin0) {
       case 1: launch_esc<2>(R, s, mode, b, grid); break;
       case 2: launch_esc<4>(R, s, mode, b, grid); break;
-      case 3: launch_esc<8>(R, s, mode, b, grid); break;
-      default: launch_esc<16>(R, s, mode, b, grid); break;
+      default: launch_esc<8>(R, s, mode, b, grid); break;  // bin 14 (bin 11 took the register sort above)
     }
     return;
   }

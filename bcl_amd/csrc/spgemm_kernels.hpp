@@ -14,14 +14,14 @@ namespace bclx {
 
 constexpr int kEmptyKey = 0x7fffffff;  // larger than any column index
 // 0: empty row; 1..9: LDS hash tables; 10: heavy (global accumulator); 11..17: sort-based (ESC) rows of up to
-// 64 * 2^(b - 11) products (bins 11..14: one wavefront per row; 15..17: 2, 4 or 8 wavefronts per row); 18: rows of
-// at most 32 products, sorted in registers by half a wavefront
+// 64 * 2^(b - 11) products (bin 11: sorted in registers by 16 lanes per row; 12..14: one wavefront per row; 15..17:
+// 2, 4 or 8 wavefronts per row); 18: rows of at most 32 products, sorted in registers by eight lanes per row
 constexpr int kNumBins = 19;
 constexpr int kHeavyBin = 10;
 constexpr int kCountStride = 32;  // bin counters sit 128 bytes apart
 constexpr int kEscBin0 = 11;
 constexpr int kEscBins = 7;
-constexpr int kEscWaveBins = 4;                         // bins 11..14 sort within one wavefront
+constexpr int kEscWaveBins = 4;                         // bins 11..14 sort within one wavefront (or less)
 constexpr int kEscMaxProducts = 64 << (kEscBins - 1);  // 4096
 constexpr int kEscColBits = 19;                         // sort keys pack column (< 2^19) and product index (13 bits)
 constexpr int kSmallBin = 18;
@@ -88,7 +88,9 @@ struct TileDev {
   const RP* b_rp;
   const int* b_col;
   const T* b_val;
-  int* c_col;  // output base (positions come from off[])
+  // output base (positions come from off[]). Every kernel writes it with nontemporal (streaming) stores: an output
+  // entry is written once and not read by this launch, so it need not displace the B rows held in L2.
+  int* c_col;
   T* c_val;
   int64_t flat0;  // first flat row of this product
   int32_t m, n;
@@ -716,8 +718,8 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
               int r = pre[w >> 3];
               for (int w2 = (w >> 3) * 8; w2 < w; w2++) r += __popc(bits[w2]);
               r += __popc(bits[w] & ((1u << (rel & 31)) - 1u));
-              tile.c_col[o + r] = rk[q];
-              tile.c_val[o + r] = (T)rv[q];
+              __builtin_nontemporal_store(rk[q], &tile.c_col[o + r]);
+              __builtin_nontemporal_store((T)rv[q], &tile.c_val[o + r]);
             }
           }
         } else {
@@ -738,8 +740,8 @@ __global__ __launch_bounds__(BLOCK) void k_hash(RowsDev<T, RP> R, int bin) {
         if (active && !ovf && !fits && lane == 0) R.stats[3] = 1ull;
         if (active && !ovf && fits) {
           for (int s = lane; s < cnt; s += G) {
-            tile.c_col[o + s] = keys[s];
-            tile.c_val[o + s] = (T)vals[s];
+            __builtin_nontemporal_store(keys[s], &tile.c_col[o + s]);
+            __builtin_nontemporal_store((T)vals[s], &tile.c_val[o + s]);
           }
         }
       }
@@ -999,8 +1001,8 @@ __global__ __launch_bounds__(64 * kEscWaves) void k_esc(RowsDev<T, RP> R, int bi
             sum += s_val[w][kp & kEscIdxMask];
           }
           const long long pos = o + before + __popcll(mask & ((1ull << lane) - 1ull));
-          tile.c_col[pos] = (int)col;
-          tile.c_val[pos] = (T)sum;
+          __builtin_nontemporal_store((int)col, &tile.c_col[pos]);
+          __builtin_nontemporal_store((T)sum, &tile.c_val[pos]);
         }
         before += __popcll(mask);
       }
@@ -1204,8 +1206,8 @@ __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& 
     for (int q = 0; q < S; q++) {
       if (head[q]) {
         const long long pos = o + before + __popc(hm[q] & ((1u << gl) - 1u));
-        tile.c_col[pos] = (int)(key[q] >> kShift);
-        tile.c_val[pos] = (T)sum[q];
+        __builtin_nontemporal_store((int)(key[q] >> kShift), &tile.c_col[pos]);
+        __builtin_nontemporal_store((T)sum[q], &tile.c_val[pos]);
       }
       before += __popc(hm[q]);
     }
@@ -1420,8 +1422,8 @@ __global__ __launch_bounds__(64 * W, 6) void k_esc_block(RowsDev<T, RP> R, int b
             sum += s_val[kp & kEscIdxMask];
           }
           const long long pos = o + before + __popcll(mask & ((1ull << lane) - 1ull));
-          tile.c_col[pos] = (int)col;
-          tile.c_val[pos] = (T)sum;
+          __builtin_nontemporal_store((int)col, &tile.c_col[pos]);
+          __builtin_nontemporal_store((T)sum, &tile.c_val[pos]);
         }
         before += __popcll(mask);
       }
@@ -1771,7 +1773,7 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
         }
         __syncthreads();
         if (s0 == 0) advance(5);
-        for (int t = tid; t < len; t += BLOCK) tile.c_col[base + s0 + t] = s_colbuf[t];
+        for (int t = tid; t < len; t += BLOCK) __builtin_nontemporal_store(s_colbuf[t], &tile.c_col[base + s0 + t]);
         __syncthreads();
         for (int t = tid; t < len; t += BLOCK) s_acc[t] = 0.0;
         __syncthreads();
@@ -1793,7 +1795,7 @@ __global__ __launch_bounds__(hr_block<MODE>()) void k_heavy_rank(RowsDev<T, RP> 
           for_each_product<BLOCK, BLOCK, true, T, RP>(tile, a0, a1, tid, s_est, s_epre, s_eval, s_lscan, nullptr, add);
           __syncthreads();
         }
-        for (int t = tid; t < len; t += BLOCK) tile.c_val[base + s0 + t] = (T)s_acc[t];
+        for (int t = tid; t < len; t += BLOCK) __builtin_nontemporal_store((T)s_acc[t], &tile.c_val[base + s0 + t]);
         __syncthreads();
       }
     } while (false);
@@ -1901,8 +1903,8 @@ __global__ __launch_bounds__(kHeavyThreads) void k_heavy(RowsDev<T, RP> R, int b
           const int j = (int)(w * 32 + bit);
           const double v = __hip_atomic_exchange(&dense[j], 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           if (fits) {
-            tile.c_col[pos] = j;
-            tile.c_val[pos] = (T)v;
+            __builtin_nontemporal_store(j, &tile.c_col[pos]);
+            __builtin_nontemporal_store((T)v, &tile.c_val[pos]);
             pos++;
           }
           word &= word - 1;
@@ -2054,15 +2056,16 @@ __global__ __launch_bounds__(kRowBlock) void k_compact(RowsDev<T, RP> R, const i
 #pragma unroll
       for (int q = 0; q < 4; q++)
         if (ok[q]) {
-          cv[q] = src_col[src[q]];
-          vv[q] = src_val[src[q]];
+          // the arena is read once: streaming loads, as the stores
+          cv[q] = __builtin_nontemporal_load(&src_col[src[q]]);
+          vv[q] = __builtin_nontemporal_load(&src_val[src[q]]);
         }
 #pragma unroll
       for (int q = 0; q < 4; q++)
         if (ok[q]) {
           const long long x = x0 + (long long)q * kRowBlock + tid;
-          dst_col[x] = cv[q];
-          dst_val[x] = vv[q];
+          __builtin_nontemporal_store(cv[q], &dst_col[x]);
+          __builtin_nontemporal_store(vv[q], &dst_val[x]);
         }
     }
     e = we;
