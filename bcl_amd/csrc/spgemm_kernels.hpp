@@ -88,8 +88,9 @@ struct TileDev {
   const RP* b_rp;
   const int* b_col;
   const T* b_val;
-  // output base (positions come from off[]). Every kernel writes it with nontemporal (streaming) stores: an output
-  // entry is written once and not read by this launch, so it need not displace the B rows held in L2.
+  // output base (positions come from off[]). The kernels write it with nontemporal (streaming) stores, except those
+  // of rows of at most 64 products: an entry is written once and not read by this launch, so it need not displace
+  // the B rows held in L2.
   int* c_col;
   T* c_val;
   int64_t flat0;  // first flat row of this product
@@ -1206,8 +1207,10 @@ __device__ inline void small_row(const RowsDev<T, RP>& R, const TileDev<T, RP>& 
     for (int q = 0; q < S; q++) {
       if (head[q]) {
         const long long pos = o + before + __popc(hm[q] & ((1u << gl) - 1u));
-        __builtin_nontemporal_store((int)(key[q] >> kShift), &tile.c_col[pos]);
-        __builtin_nontemporal_store((T)sum[q], &tile.c_val[pos]);
+        // plain stores here: rows this small write a fraction of a cache line each, and L2 merges neighbouring
+        // rows' pieces before they reach HBM (streaming stores made a 4096^2-grid Laplacian 6.6 -> 7.2 ms)
+        tile.c_col[pos] = (int)(key[q] >> kShift);
+        tile.c_val[pos] = (T)sum[q];
       }
       before += __popc(hm[q]);
     }
